@@ -1,0 +1,227 @@
+"""canvas_ity_amd -- Python bindings to libcanvas_ity_hip.so (the C ABI in
+include/cih_abi.h) for tests, benchmarks and the serving front end.
+
+The product is the C++ header include/canvas_ity_hip.hpp plus the HIP
+library; this module only drives the library through its C ABI with ctypes.
+Nothing here computes pixels: every call goes to the GPU, and loading fails
+loudly when the library has not been built (python -m canvas_ity_amd.build).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libcanvas_ity_hip.so")
+
+OK = 0
+ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "device error",
+          -4: "malformed input", -5: "over memory budget"}
+
+# composite_operation values (canvas_ity.hpp:146-149)
+OPS = {"source_in": 1, "source_copy": 2, "source_out": 3, "destination_in": 4,
+       "destination_atop": 7, "lighter": 10, "destination_over": 11,
+       "destination_out": 12, "source_atop": 13, "source_over": 14, "exclusive_or": 15}
+
+# Every function declared in include/cih_abi.h, with its ctypes signature.
+_c = ctypes
+_P = _c.c_void_p
+SIGNATURES = {
+    "cih_device_open": (_c.c_int, [_c.c_int, _c.POINTER(_P)]),
+    "cih_device_close": (_c.c_int, [_P]),
+    "cih_device_synchronize": (_c.c_int, [_P]),
+    "cih_last_error": (_c.c_char_p, []),
+    "cih_canvas_create": (_c.c_int, [_P, _c.c_int, _c.c_int, _c.POINTER(_P)]),
+    "cih_canvas_destroy": (_c.c_int, [_P]),
+    "cih_canvas_size": (_c.c_int, [_P, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
+    "cih_canvas_device": (_c.c_int, [_P, _c.POINTER(_P)]),
+    "cih_image_create": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int,
+                                    _c.POINTER(_c.c_int32)]),
+    "cih_canvas_submit": (_c.c_int, [_P, _P, _c.c_size_t]),
+    "cih_canvas_flush": (_c.c_int, [_P]),
+    "cih_canvas_get_image_data": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int,
+                                             _c.c_int, _c.c_int]),
+    "cih_canvas_put_image_data": (_c.c_int, [_P, _P, _c.c_int, _c.c_int, _c.c_int,
+                                             _c.c_int, _c.c_int]),
+    "cih_canvas_read_linear": (_c.c_int, [_P, _P]),
+    "cih_canvas_replay": (_c.c_int, [_P, _P, _c.c_size_t]),
+    "cih_trace_polygons": (_c.c_int, [_P, _c.c_int, _P, _c.c_size_t, _P, _c.c_size_t,
+                                      _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)]),
+    "cih_trace_runs": (_c.c_int, [_P, _c.c_int, _c.c_int, _P, _P, _P, _c.c_size_t,
+                                  _c.POINTER(_c.c_size_t)]),
+    "cih_scene_generate": (_c.c_int, [_c.c_int, _c.c_uint64, _c.c_int, _c.c_int, _c.c_int,
+                                      _c.c_int, _c.POINTER(_P), _c.POINTER(_c.c_size_t)]),
+    "cih_scene_seed": (_c.c_uint64, [_c.c_uint64, _c.c_uint64]),
+    "cih_free": (None, [_P]),
+}
+
+_lib = None
+
+
+def library():
+    """Loads libcanvas_ity_hip.so; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise RuntimeError("libcanvas_ity_hip.so is not built: run "
+                               "python -m canvas_ity_amd.build (needs hipcc)")
+        lib = ctypes.CDLL(LIBRARY)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class CihError(RuntimeError):
+    pass
+
+
+def _check(status, what):
+    if status != OK:
+        msg = library().cih_last_error().decode(errors="replace")
+        raise CihError("%s failed (%s): %s" % (what, ERRORS.get(status, status), msg))
+
+
+def scene(config, seed, width=0, height=0, shapes=0, flags=0):
+    """The seeded command stream of a benchmark configuration (bytes)."""
+    lib = library()
+    out = _P()
+    size = _c.c_size_t()
+    _check(lib.cih_scene_generate(config, seed, width, height, shapes, flags,
+                                  _c.byref(out), _c.byref(size)), "cih_scene_generate")
+    try:
+        return _c.string_at(out, size.value)
+    finally:
+        lib.cih_free(out)
+
+
+def scene_seed(seed, index):
+    return library().cih_scene_seed(seed, index)
+
+
+SCENE_EVENODD, SCENE_NO_IMAGES, SCENE_SHADOWS = 1, 2, 4
+
+
+def stream_size(stream):
+    words = np.frombuffer(stream[:16], dtype=np.uint32)
+    return int(words[2]), int(words[3])
+
+
+class Device:
+    def __init__(self, ordinal=0):
+        self.handle = _P()
+        _check(library().cih_device_open(ordinal, _c.byref(self.handle)), "cih_device_open")
+
+    def synchronize(self):
+        _check(library().cih_device_synchronize(self.handle), "cih_device_synchronize")
+
+    def close(self):
+        if self.handle:
+            library().cih_device_close(self.handle)
+            self.handle = _P()
+
+
+_default = None
+
+
+def default_device():
+    global _default
+    if _default is None:
+        _default = Device(0)
+    return _default
+
+
+class Canvas:
+    """A device canvas driven through the C ABI."""
+
+    def __init__(self, width, height, device=None):
+        self.device = device or default_device()
+        self.width, self.height = width, height
+        self.handle = _P()
+        _check(library().cih_canvas_create(self.device.handle, width, height,
+                                           _c.byref(self.handle)), "cih_canvas_create")
+
+    def replay(self, stream):
+        buf = _c.create_string_buffer(bytes(stream), len(stream))
+        _check(library().cih_canvas_replay(self.handle, buf, len(stream)), "cih_canvas_replay")
+
+    def flush(self):
+        _check(library().cih_canvas_flush(self.handle), "cih_canvas_flush")
+
+    def get_image_data(self, width=None, height=None, stride=None, x=0, y=0, out=None):
+        width = self.width if width is None else width
+        height = self.height if height is None else height
+        stride = 4 * width if stride is None else stride
+        if out is None:
+            out = np.zeros((height, stride), dtype=np.uint8)
+        _check(library().cih_canvas_get_image_data(
+            self.handle, out.ctypes.data_as(_P), width, height, stride, x, y),
+            "cih_canvas_get_image_data")
+        return out
+
+    def rgba8(self):
+        return self.get_image_data().reshape(self.height, self.width, 4)
+
+    def linear(self):
+        out = np.zeros((self.height, self.width, 4), dtype=np.float32)
+        _check(library().cih_canvas_read_linear(self.handle, out.ctypes.data_as(_P)),
+               "cih_canvas_read_linear")
+        return out
+
+    def put_image_data(self, image, x=0, y=0):
+        image = np.ascontiguousarray(image, dtype=np.uint8)
+        h, w = image.shape[0], image.shape[1]
+        _check(library().cih_canvas_put_image_data(self.handle, image.ctypes.data_as(_P),
+                                                   w, h, 4 * w, x, y),
+               "cih_canvas_put_image_data")
+
+    def trace_polygons(self, draw):
+        """(points [n,2], subpaths [m,2]) that draw `draw` of the last frame made."""
+        lib = library()
+        n, m = _c.c_size_t(), _c.c_size_t()
+        _check(lib.cih_trace_polygons(self.handle, draw, None, 0, None, 0, _c.byref(n),
+                                      _c.byref(m)), "cih_trace_polygons")
+        pts = np.zeros((n.value, 2), dtype=np.float32)
+        sub = np.zeros((m.value, 2), dtype=np.uint32)
+        _check(lib.cih_trace_polygons(self.handle, draw, pts.ctypes.data_as(_P), n.value,
+                                      sub.ctypes.data_as(_P), m.value, _c.byref(n),
+                                      _c.byref(m)), "cih_trace_polygons")
+        return pts, sub
+
+    def trace_runs(self, draw, shadow=False):
+        """(x, y, delta) arrays of the draw's runs in the last frame, unsorted."""
+        lib = library()
+        n = _c.c_size_t()
+        _check(lib.cih_trace_runs(self.handle, draw, int(shadow), None, None, None, 0,
+                                  _c.byref(n)), "cih_trace_runs")
+        xs = np.zeros(n.value, dtype=np.int32)
+        ys = np.zeros(n.value, dtype=np.int32)
+        ds = np.zeros(n.value, dtype=np.float32)
+        _check(lib.cih_trace_runs(self.handle, draw, int(shadow), xs.ctypes.data_as(_P),
+                                  ys.ctypes.data_as(_P), ds.ctypes.data_as(_P), n.value,
+                                  _c.byref(n)), "cih_trace_runs")
+        return xs, ys, ds
+
+    def close(self):
+        if self.handle:
+            library().cih_canvas_destroy(self.handle)
+            self.handle = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render(stream, device=None):
+    """Replays a command stream on a fresh canvas; returns (sRGB8, linear)."""
+    w, h = stream_size(stream)
+    canvas = Canvas(w, h, device)
+    try:
+        canvas.replay(stream)
+        return canvas.rgba8(), canvas.linear()
+    finally:
+        canvas.close()

@@ -1,0 +1,101 @@
+// cih_device.hpp -- data shared by the kernels (cih_kernels.hip) and the
+// host runtime (cih_runtime.cpp).  Plain structs, no HIP types.
+#ifndef CIH_DEVICE_HPP
+#define CIH_DEVICE_HPP
+
+#include <stdint.h>
+
+namespace cih {
+
+// One draw as the kernels see it, decoded from a cih_draw_packet by the host.
+// Offsets are in bytes from the start of the frame's payload buffer.
+struct draw_desc {
+    int32_t kind, op, flags, canvas;
+    float global_alpha, line_width, miter_limit, dash_offset;
+    int32_t line_cap, line_join;
+    float forward[6], inverse[6], paint_inverse[6];
+    float shadow_color[4];
+    float shadow_offset_x, shadow_offset_y;
+    int32_t shadow; // 1 when render_shadow runs (hpp:2398-2401)
+    int32_t radius, border;
+    float weight_1, weight_2;
+    int32_t brush;
+    float start_x, start_y, end_x, end_y, start_radius, end_radius;
+    int32_t image, image_width, image_height, repetition;
+    uint32_t points_at, npoints, subpaths_at, nsubpaths;
+    uint32_t dash_at, ndash, colors_at, ncolors, stops_at, nstops;
+};
+
+// Results the kernels leave for the later stages of the same frame.
+struct draw_state {
+    // final polygons: device addresses of xy points and (count, closed) pairs
+    uint64_t poly_points, poly_counts;
+    uint32_t poly_npoints, poly_nsub;
+    // main scan conversion: rows [row0, row0 + rows); run offsets per row
+    int32_t row0, rows;
+    uint64_t rowoff; // word-arena index of rows + 1 offsets
+    uint64_t runs;   // run-arena index of the first run
+    int32_t minx, maxx;
+    // shadow scan conversion (extended canvas coordinates)
+    int32_t srow0, srows;
+    uint64_t srowoff, sruns;
+    // shadow image: [left, left + width) x [top, top + height) of the
+    // extended canvas, stored row-major at `image` (and `image2` ping-pong)
+    int32_t left, top, width, height;
+    uint64_t image, image2;
+};
+
+struct image_desc {
+    uint64_t texels; // device address of float4 premultiplied linear texels
+    int32_t width, height;
+};
+
+struct canvas_desc {
+    uint64_t bitmap; // device address of float4 pixels
+    int32_t width, height;
+    int32_t first_draw, ndraws;
+};
+
+struct run {
+    int32_t x;
+    float delta;
+};
+
+// Arena counters (device, 64-bit): requested amounts keep counting past the
+// capacity so the host learns how much a frame needed.
+enum counter {
+    COUNTER_POINTS = 0,
+    COUNTER_WORDS = 1,
+    COUNTER_RUNS = 2,
+    COUNTER_SHADOW = 3,
+    COUNTER_OVERFLOW = 4,
+    COUNTER_QUEUE = 5,
+    COUNTER_COUNT = 8
+};
+
+struct frame {
+    const unsigned char *payload;
+    const draw_desc *draws;
+    draw_state *state;
+    const canvas_desc *canvases;
+    const image_desc *images;
+    int32_t ndraws, ncanvases;
+    // arenas
+    float *points; // xy pairs
+    uint64_t points_cap;
+    uint32_t *words;
+    uint64_t words_cap;
+    run *runs;
+    uint64_t runs_cap;
+    float *shadow;
+    uint64_t shadow_cap;
+    unsigned long long *counters;
+    // fine-stage work list: (canvas, row, segment) items
+    const uint32_t *items;
+    uint32_t nitems;
+    int32_t segment; // pixels per fine work item
+};
+
+} // namespace cih
+
+#endif // CIH_DEVICE_HPP

@@ -1,0 +1,472 @@
+// cih_geometry.hpp -- per-thread geometry stages of the rasteriser.
+//
+// Flattening (canvas_ity.hpp:1331-1524), dashing (hpp:1858-1934), stroke
+// expansion (hpp:1949-2100), polygon clipping (hpp:2205-2238) and signed-area
+// scan conversion (hpp:2109-2170), written as pure functions over "sinks" so
+// one body serves a counting pass and an emitting pass.  The kernels in
+// cih_device.hip run one thread per cubic, per subpath or per edge.
+//
+// Every flattening, dashing and stroking decision is a threshold test on float
+// arithmetic, so each expression keeps the reference's operand order and
+// association, no multiply-add is contracted (the library is built with
+// -ffp-contract=off), and min/max keep std::min/std::max's NaN behaviour.
+#ifndef CIH_GEOMETRY_HPP
+#define CIH_GEOMETRY_HPP
+
+#include <math.h>
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define CIH_HD __host__ __device__ inline
+#else
+#define CIH_HD inline
+#endif
+
+namespace cih {
+
+struct f2 {
+    float x, y;
+};
+
+CIH_HD f2 mk(float x, float y) {
+    f2 v;
+    v.x = x;
+    v.y = y;
+    return v;
+}
+CIH_HD f2 add(f2 a, f2 b) { return mk(a.x + b.x, a.y + b.y); }
+CIH_HD f2 sub(f2 a, f2 b) { return mk(a.x - b.x, a.y - b.y); }
+CIH_HD f2 mul(float s, f2 v) { return mk(v.x * s, v.y * s); }
+CIH_HD float dot(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
+CIH_HD float len(f2 v) { return sqrtf(dot(v, v)); }
+CIH_HD float smax(float a, float b) { return (a < b) ? b : a; } // std::max
+CIH_HD float smin(float a, float b) { return (b < a) ? b : a; } // std::min
+CIH_HD f2 normalized(f2 v) { return mul(1.0f / smax(1.0e-6f, len(v)), v); }
+CIH_HD f2 perp(f2 v) { return mk(-v.y, v.x); }
+CIH_HD f2 lerp(f2 from, f2 to, float ratio) { return add(from, mul(ratio, sub(to, from))); }
+// Affine a,b,c,d,e,f applied as (a*x + c*y) + e, (b*x + d*y) + f.
+CIH_HD f2 apply(const float *m, f2 v) {
+    return mk(m[0] * v.x + m[2] * v.y + m[4], m[1] * v.x + m[3] * v.y + m[5]);
+}
+
+// ---------------------------------------------------------------------------
+// Flattening.  A point sink needs `void point(f2)`.
+
+// Adaptive subdivision of one cubic (hpp:1331-1387), iterative and
+// depth-first so leaves come out in the recursion's order.  Flatness 0.125 px,
+// at most 20 levels; `angular` > -1 adds the turn-cosine test and makes leaves
+// emit their control points (stroking).
+template <class Sink>
+CIH_HD void tessellate(Sink &sink, f2 p1, f2 c1, f2 c2, f2 p2, float angular) {
+    struct node {
+        f2 p1, c1, c2, p2;
+        int limit;
+    };
+    node stack[22];
+    int top = 0;
+    stack[top].p1 = p1, stack[top].c1 = c1, stack[top].c2 = c2, stack[top].p2 = p2;
+    stack[top++].limit = 20;
+    const float flatness = 0.125f * 0.125f;
+    while (top) {
+        node n = stack[--top];
+        f2 edge_1 = sub(n.c1, n.p1);
+        f2 edge_2 = sub(n.c2, n.c1);
+        f2 edge_3 = sub(n.p2, n.c2);
+        f2 segment = sub(n.p2, n.p1);
+        float squared_1 = dot(edge_1, edge_1);
+        float squared_2 = dot(edge_2, edge_2);
+        float squared_3 = dot(edge_3, edge_3);
+        float length_squared = smax(1.0e-4f, dot(segment, segment));
+        float projection_1 = dot(edge_1, segment) / length_squared;
+        float projection_2 = dot(edge_3, segment) / length_squared;
+        float clamped_1 = smin(smax(projection_1, 0.0f), 1.0f);
+        float clamped_2 = smin(smax(projection_2, 0.0f), 1.0f);
+        f2 to_line_1 = sub(add(n.p1, mul(clamped_1, segment)), n.c1);
+        f2 to_line_2 = sub(sub(n.p2, mul(clamped_2, segment)), n.c2);
+        float cosine = 1.0f;
+        if (angular > -1.0f) {
+            if (squared_1 * squared_3 != 0.0f)
+                cosine = dot(edge_1, edge_3) / sqrtf(squared_1 * squared_3);
+            else if (squared_1 * squared_2 != 0.0f)
+                cosine = dot(edge_1, edge_2) / sqrtf(squared_1 * squared_2);
+            else if (squared_2 * squared_3 != 0.0f)
+                cosine = dot(edge_2, edge_3) / sqrtf(squared_2 * squared_3);
+        }
+        if ((dot(to_line_1, to_line_1) <= flatness && dot(to_line_2, to_line_2) <= flatness &&
+             cosine >= angular) ||
+            !n.limit) {
+            if (angular > -1.0f && squared_1 != 0.0f)
+                sink.point(n.c1);
+            if (angular > -1.0f && squared_2 != 0.0f)
+                sink.point(n.c2);
+            if (angular == -1.0f || squared_3 != 0.0f)
+                sink.point(n.p2);
+            continue;
+        }
+        f2 left_1 = lerp(n.p1, n.c1, 0.5f);
+        f2 middle = lerp(n.c1, n.c2, 0.5f);
+        f2 right_2 = lerp(n.c2, n.p2, 0.5f);
+        f2 left_2 = lerp(left_1, middle, 0.5f);
+        f2 right_1 = lerp(middle, right_2, 0.5f);
+        f2 split = lerp(left_2, right_1, 0.5f);
+        // right half below the left half: the left is finished first
+        stack[top].p1 = split, stack[top].c1 = right_1, stack[top].c2 = right_2;
+        stack[top].p2 = n.p2, stack[top++].limit = n.limit - 1;
+        stack[top].p1 = n.p1, stack[top].c1 = left_1, stack[top].c2 = left_2;
+        stack[top].p2 = split, stack[top++].limit = n.limit - 1;
+    }
+}
+
+// Cuts a cubic at its x/y extrema and curvature extremum, then tessellates
+// each piece (hpp:1398-1487).
+template <class Sink>
+CIH_HD void add_bezier(Sink &sink, f2 point_1, f2 control_1, f2 control_2, f2 point_2,
+                       float angular) {
+    f2 edge_1 = sub(control_1, point_1);
+    f2 edge_2 = sub(control_2, control_1);
+    f2 edge_3 = sub(point_2, control_2);
+    if (dot(edge_1, edge_1) == 0.0f && dot(edge_3, edge_3) == 0.0f) {
+        sink.point(point_2);
+        return;
+    }
+    float at[7] = {0.0f, 1.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    int cuts = 2;
+    f2 extrema_a = add(mul(-9.0f, edge_2), mul(3.0f, sub(point_2, point_1)));
+    f2 extrema_b = sub(mul(6.0f, add(point_1, control_2)), mul(12.0f, control_1));
+    f2 extrema_c = mul(3.0f, edge_1);
+    const float epsilon = 1.0e-4f;
+    for (int axis = 0; axis < 2; ++axis) {
+        float a = axis ? extrema_a.y : extrema_a.x;
+        float b = axis ? extrema_b.y : extrema_b.x;
+        float c = axis ? extrema_c.y : extrema_c.x;
+        if (fabsf(a) > epsilon) {
+            float discriminant = b * b - 4.0f * a * c;
+            if (discriminant >= 0.0f) {
+                float sign = b > 0.0f ? 1.0f : -1.0f;
+                float term = -b - sign * sqrtf(discriminant);
+                float extremum_1 = term / (2.0f * a);
+                at[cuts++] = extremum_1;
+                at[cuts++] = c / (a * extremum_1);
+            }
+        } else if (fabsf(b) > epsilon)
+            at[cuts++] = -c / b;
+    }
+    float determinant_1 = dot(perp(edge_1), edge_2);
+    float determinant_2 = dot(perp(edge_1), edge_3);
+    float determinant_3 = dot(perp(edge_2), edge_3);
+    float curve_a = determinant_1 - determinant_2 + determinant_3;
+    float curve_b = -2.0f * determinant_1 + determinant_2;
+    if (fabsf(curve_a) > epsilon && fabsf(curve_b) > epsilon)
+        at[cuts++] = -0.5f * curve_b / curve_a;
+    for (int index = 1; index < cuts; ++index) { // insertion sort, as std order
+        float value = at[index];
+        int sorted = index - 1;
+        for (; 0 <= sorted && value < at[sorted]; --sorted)
+            at[sorted + 1] = at[sorted];
+        at[sorted + 1] = value;
+    }
+    f2 split_point_1 = point_1;
+    for (int index = 0; index < cuts - 1; ++index) {
+        if (!(0.0f <= at[index] && at[index + 1] <= 1.0f && at[index] != at[index + 1]))
+            continue;
+        float t = at[index + 1];
+        float ratio = at[index] / t;
+        f2 partial_1 = lerp(point_1, control_1, t);
+        f2 partial_2 = lerp(control_1, control_2, t);
+        f2 partial_3 = lerp(control_2, point_2, t);
+        f2 partial_4 = lerp(partial_1, partial_2, t);
+        f2 partial_5 = lerp(partial_2, partial_3, t);
+        f2 partial_6 = lerp(partial_1, partial_4, ratio);
+        f2 split_point_2 = lerp(partial_4, partial_5, t);
+        f2 split_control_2 = lerp(partial_4, split_point_2, ratio);
+        f2 split_control_1 = lerp(partial_6, split_control_2, ratio);
+        tessellate(sink, split_point_1, split_control_1, split_control_2, split_point_2,
+                   angular);
+        split_point_1 = split_point_2;
+    }
+}
+
+// Turn-cosine limit for stroking (hpp:1497-1500); -1 (no limit) for fills.
+CIH_HD float angular_limit(bool stroking, float line_width) {
+    const float tolerance = 0.125f;
+    float ratio = tolerance / smax(0.5f * line_width, tolerance);
+    return stroking ? (ratio - 2.0f) * ratio * 2.0f + 1.0f : -1.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Line-path sinks need `void point(f2)` and `void subpath(uint32_t count,
+// bool closed)` plus the running counts `points` and `subpaths`.  Dashing
+// also needs `void close_last()` and `void merge_last(uint32_t merge_point,
+// uint32_t merge_subpath)` (the std::rotate merge of hpp:1916-1931).
+
+struct dash_setup {
+    int start;
+    float offset;
+};
+
+// Dash phase shared by all subpaths (hpp:1866-1876).  The phase walk is
+// bounded so a degenerate pattern cannot spin a GPU thread forever.
+CIH_HD dash_setup dash_prepare(const float *dash, int count, float dash_offset) {
+    float total = 0.0f;
+    for (int k = 0; k < count; ++k)
+        total = total + dash[k];
+    float offset = fmodf(dash_offset, total);
+    if (offset < 0.0f)
+        offset += total;
+    int start = 0;
+    for (int guard = 0; offset >= dash[start] && guard < (1 << 20); ++guard) {
+        offset -= dash[start];
+        start = start + 1 < count ? start + 1 : 0;
+    }
+    dash_setup s;
+    s.start = start;
+    s.offset = offset;
+    return s;
+}
+
+// Dashes one subpath of `count` points (hpp:1877-1933).
+template <class Sink>
+CIH_HD void dash_subpath(Sink &out, const f2 *points, uint32_t count, bool closed,
+                         const float *dash, int ndash, dash_setup setup,
+                         const float *inverse) {
+    uint32_t index = 0;
+    uint32_t first = out.points;
+    int segment = setup.start;
+    bool emit = (~setup.start & 1) != 0;
+    uint32_t merge_point = out.points;
+    uint32_t merge_subpath = out.subpaths;
+    bool merge_emit = emit;
+    float next = dash[setup.start] - setup.offset;
+    for (; index + 1 < count; ++index) {
+        f2 from = points[index];
+        f2 to = points[index + 1];
+        if (emit)
+            out.point(from);
+        float line = len(sub(apply(inverse, to), apply(inverse, from)));
+        for (int guard = 0; next < line && guard < (1 << 24); ++guard) {
+            out.point(lerp(from, to, next / line));
+            if (emit) {
+                out.subpath(out.points - first, false);
+                first = out.points;
+            }
+            segment = segment + 1 < ndash ? segment + 1 : 0;
+            emit = !emit;
+            next += dash[segment];
+        }
+        next -= line;
+    }
+    if (emit) {
+        out.point(points[index]);
+        out.subpath(out.points - first, false);
+        if (closed && merge_emit) {
+            if (out.subpaths == merge_subpath + 1)
+                out.close_last();
+            else
+                out.merge_last(merge_point, merge_subpath);
+        }
+    }
+}
+
+// One side of a stroke (hpp:1949-2058): walks points[beginning..ending] in
+// either direction in user space (through `inverse`), adding joins and, for
+// open subpaths, the end cap, as device-space points (through `forward`).
+template <class Sink>
+CIH_HD void half_stroke(Sink &out, const f2 *points, uint32_t beginning, uint32_t ending,
+                        bool closed, float line_width, float miter_limit, int join, int cap,
+                        const float *forward, const float *inverse) {
+    float half = line_width * 0.5f;
+    float ratio = miter_limit * miter_limit * half * half;
+    f2 in_direction = mk(0.0f, 0.0f);
+    float in_length = 0.0f;
+    f2 point = apply(inverse, points[beginning]);
+    uint32_t finish = beginning;
+    uint32_t index = beginning;
+    const float epsilon = 1.0e-4f;
+    do {
+        f2 next = apply(inverse, points[index]);
+        f2 out_direction = normalized(sub(next, point));
+        float out_length = len(sub(next, point));
+        if (in_length != 0.0f && out_length >= epsilon) {
+            if (closed && finish == beginning)
+                finish = index;
+            f2 side_in = add(point, mul(half, perp(in_direction)));
+            f2 side_out = add(point, mul(half, perp(out_direction)));
+            float turn = dot(perp(in_direction), out_direction);
+            if (fabsf(turn) < epsilon)
+                turn = 0.0f;
+            f2 offset = turn == 0.0f ? mk(0.0f, 0.0f)
+                                     : mul(half / turn, sub(out_direction, in_direction));
+            bool tight = (dot(offset, in_direction) < -in_length &&
+                          dot(offset, out_direction) > out_length);
+            if (turn > 0.0f && tight) {
+                f2 t = side_in;
+                side_in = side_out;
+                side_out = t;
+                t = in_direction;
+                in_direction = out_direction;
+                out_direction = t;
+                out.point(apply(forward, side_out));
+                out.point(apply(forward, point));
+                out.point(apply(forward, side_in));
+            }
+            if ((turn > 0.0f && !tight) ||
+                (turn != 0.0f && join == 0 && dot(offset, offset) <= ratio))
+                out.point(apply(forward, add(point, offset)));
+            else if (join == 2) {
+                float cosine = dot(in_direction, out_direction);
+                float angle = acosf(smin(smax(cosine, -1.0f), 1.0f));
+                float alpha = 4.0f / 3.0f * tanf(0.25f * angle);
+                out.point(apply(forward, side_in));
+                add_bezier(out, apply(forward, side_in),
+                           apply(forward, add(side_in, mul(alpha * half, in_direction))),
+                           apply(forward, sub(side_out, mul(alpha * half, out_direction))),
+                           apply(forward, side_out), -1.0f);
+            } else {
+                out.point(apply(forward, side_in));
+                out.point(apply(forward, side_out));
+            }
+            if (turn > 0.0f && tight) {
+                out.point(apply(forward, side_out));
+                out.point(apply(forward, point));
+                out.point(apply(forward, side_in));
+                f2 t = in_direction;
+                in_direction = out_direction;
+                out_direction = t;
+            }
+        }
+        if (out_length >= epsilon) {
+            in_direction = out_direction;
+            in_length = out_length;
+            point = next;
+        }
+        index = (index == ending ? beginning : ending > beginning ? index + 1 : index - 1);
+    } while (index != finish);
+    if (closed || in_length == 0.0f)
+        return;
+    f2 ahead = mul(half, in_direction);
+    f2 side = perp(ahead);
+    if (cap == 0) {
+        out.point(apply(forward, add(point, side)));
+        out.point(apply(forward, sub(point, side)));
+    } else if (cap == 1) {
+        out.point(apply(forward, add(add(point, ahead), side)));
+        out.point(apply(forward, sub(add(point, ahead), side)));
+    } else if (cap == 2) {
+        const float alpha = 0.55228475f; // 4/3*tan(pi/8)
+        out.point(apply(forward, add(point, side)));
+        add_bezier(out, apply(forward, add(point, side)),
+                   apply(forward, add(add(point, side), mul(alpha, ahead))),
+                   apply(forward, add(add(point, ahead), mul(alpha, side))),
+                   apply(forward, add(point, ahead)), -1.0f);
+        add_bezier(out, apply(forward, add(point, ahead)),
+                   apply(forward, sub(add(point, ahead), mul(alpha, side))),
+                   apply(forward, add(sub(point, side), mul(alpha, ahead))),
+                   apply(forward, sub(point, side)), -1.0f);
+    }
+}
+
+// Stroke outline of one subpath (hpp:2081-2098): one closed loop for an open
+// subpath, an outer and an inner loop for a closed one.
+template <class Sink>
+CIH_HD void stroke_subpath(Sink &out, const f2 *points, uint32_t count, bool closed,
+                           float line_width, float miter_limit, int join, int cap,
+                           const float *forward, const float *inverse) {
+    if (count < 2)
+        return;
+    uint32_t first = out.points;
+    half_stroke(out, points, 0, count - 1, closed, line_width, miter_limit, join, cap,
+                forward, inverse);
+    if (closed) {
+        out.subpath(out.points - first, true);
+        first = out.points;
+    }
+    half_stroke(out, points, count - 1, 0, closed, line_width, miter_limit, join, cap,
+                forward, inverse);
+    out.subpath(out.points - first, true);
+}
+
+// ---------------------------------------------------------------------------
+// Clipping: one Sutherland-Hodgman stage for the half-plane of `edge` (0: x>=0,
+// 1: y>=0, 2: x<=width, 3: y<=height), for the polygon edge from -> to
+// (hpp:2210-2225).  Emits up to two points.
+template <class Sink>
+CIH_HD void clip_edge(Sink &out, f2 from, f2 to, int edge, float width, float height) {
+    f2 normal = mk(edge == 0 ? 1.0f : edge == 2 ? -1.0f : 0.0f,
+                   edge == 1 ? 1.0f : edge == 3 ? -1.0f : 0.0f);
+    float place = edge == 2 ? width : edge == 3 ? height : 0.0f;
+    float from_side = dot(from, normal) + place;
+    float to_side = dot(to, normal) + place;
+    if (from_side * to_side < 0.0f)
+        out.point(lerp(from, to, from_side / (from_side - to_side)));
+    if (to_side >= 0.0f)
+        out.point(to);
+}
+
+CIH_HD bool clip_inside(f2 p, float width, float height) {
+    // the four stage tests of clip_edge for a single point
+    return p.x * 1.0f + p.y * 0.0f + 0.0f >= 0.0f && p.x * 0.0f + p.y * 1.0f + 0.0f >= 0.0f &&
+           p.x * -1.0f + p.y * 0.0f + width >= 0.0f && p.x * 0.0f + p.y * -1.0f + height >= 0.0f;
+}
+
+CIH_HD f2 clamp_point(f2 p, float width, float height) {
+    return mk(smin(smax(p.x, 0.0f), width), smin(smax(p.y, 0.0f), height));
+}
+
+// ---------------------------------------------------------------------------
+// Scan conversion of one edge into signed coverage changes (hpp:2109-2170).
+// A run sink needs `void run(int x, int y, float delta)`; runs arrive row by
+// row (y monotone) and left to right within a row.  Loops are bounded by the
+// clamped coordinates; the guards only stop malformed input.
+template <class Sink>
+CIH_HD void add_runs(Sink &sink, f2 from, f2 to) {
+    const float epsilon = 2.0e-5f;
+    if (fabsf(to.y - from.y) < epsilon)
+        return;
+    float sign = to.y > from.y ? 1.0f : -1.0f;
+    if (from.x > to.x) {
+        f2 t = from;
+        from = to;
+        to = t;
+    }
+    f2 now = from;
+    f2 pixel = mk(floorf(now.x), floorf(now.y));
+    f2 corner = add(pixel, mk(1.0f, to.y > from.y ? 1.0f : 0.0f));
+    f2 slope = mk((to.x - from.x) / (to.y - from.y), (to.y - from.y) / (to.x - from.x));
+    f2 next_x = (to.x - from.x < epsilon) ? to
+                                          : mk(corner.x, now.y + (corner.x - now.x) * slope.y);
+    f2 next_y = mk(now.x + (corner.y - now.y) * slope.x, corner.y);
+    if ((from.y < to.y && to.y < next_y.y) || (from.y > to.y && to.y > next_y.y))
+        next_y = to;
+    float y_step = to.y > from.y ? 1.0f : -1.0f;
+    int rows = 0;
+    do {
+        float carry = 0.0f;
+        int steps = 0;
+        while (next_x.x < next_y.x && steps++ < (1 << 17)) {
+            float strip = smin(smax((next_x.y - now.y) * y_step, 0.0f), 1.0f);
+            float mid = (next_x.x + now.x) * 0.5f;
+            float area = (mid - pixel.x) * strip;
+            sink.run((int)pixel.x, (int)pixel.y, (carry + strip - area) * sign);
+            carry = area;
+            now = next_x;
+            next_x.x += 1.0f;
+            next_x.y = (next_x.x - from.x) * slope.y + from.y;
+            pixel.x += 1.0f;
+        }
+        float strip = smin(smax((next_y.y - now.y) * y_step, 0.0f), 1.0f);
+        float mid = (next_y.x + now.x) * 0.5f;
+        float area = (mid - pixel.x) * strip;
+        sink.run((int)pixel.x, (int)pixel.y, (carry + strip - area) * sign);
+        sink.run((int)(pixel.x + 1.0f), (int)pixel.y, area * sign);
+        now = next_y;
+        next_y.y += y_step;
+        next_y.x = (next_y.y - from.y) * slope.x + from.x;
+        pixel.y += y_step;
+        if ((from.y < to.y && to.y < next_y.y) || (from.y > to.y && to.y > next_y.y))
+            next_y = to;
+    } while (now.y != to.y && ++rows < (1 << 17));
+}
+
+} // namespace cih
+
+#endif // CIH_GEOMETRY_HPP

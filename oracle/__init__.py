@@ -1,0 +1,147 @@
+"""TEST INFRASTRUCTURE ONLY -- the checker side of the parity tests.
+
+Two CPU implementations of the canvas pipeline, loaded with ctypes:
+
+* ``port``      liboracle.so, the C restatement in cih_oracle.c (each
+                function cites the canvas_ity.hpp lines it follows);
+* ``reference`` _ref/libcih_ref.so, the reference header itself compiled
+                from its source by oracle/Makefile (absent where the
+                reference checkout and its prebuilt file are both missing).
+
+Both replay the same command streams (include/cih_stream.h) as the HIP
+library.  Only tests/, __graft_entry__.smoke() and bench.py's CPU baseline
+use this package; the product never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PORT = os.path.join(HERE, "liboracle.so")
+REFERENCE = os.path.join(HERE, "_ref", "libcih_ref.so")
+
+_P = ctypes.c_void_p
+_libs = {}
+
+
+def build():
+    """Builds liboracle.so (and _ref/ when the reference checkout exists)."""
+    import subprocess
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def _load(kind):
+    if kind in _libs:
+        return _libs[kind]
+    path, prefix = (PORT, "cih_oracle") if kind == "port" else (REFERENCE, "cih_ref")
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    lib = ctypes.CDLL(path)
+    fns = {}
+    for name, res, args in (
+            ("create", _P, [ctypes.c_int, ctypes.c_int]),
+            ("destroy", None, [_P]),
+            ("replay", ctypes.c_int, [_P, _P, ctypes.c_size_t]),
+            ("get_image_data", None, [_P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int]),
+            ("read_linear", None, [_P, _P])):
+        fn = getattr(lib, prefix + "_" + name)
+        fn.restype = res
+        fn.argtypes = args
+        fns[name] = fn
+    if kind == "reference":
+        fn = lib.cih_ref_path_lines
+        fn.restype = ctypes.c_long
+        fn.argtypes = [_P, ctypes.c_int, _P, ctypes.c_long, _P, ctypes.c_long,
+                       ctypes.POINTER(ctypes.c_long)]
+        fn = lib.cih_ref_lines_to_runs
+        fn.restype = ctypes.c_long
+        fn.argtypes = [_P, ctypes.c_float, ctypes.c_float, ctypes.c_int, _P, ctypes.c_long]
+    fns["lib"] = lib
+    _libs[kind] = fns
+    return fns
+
+
+def available(kind):
+    try:
+        _load(kind)
+        return True
+    except (FileNotFoundError, OSError):
+        return False
+
+
+class Canvas:
+    """A CPU canvas of the given kind ('port' or 'reference')."""
+
+    def __init__(self, width, height, kind="port"):
+        self.fns = _load(kind)
+        self.width, self.height = width, height
+        self.handle = self.fns["create"](width, height)
+        if not self.handle:
+            raise ValueError("invalid canvas size")
+
+    def replay(self, stream):
+        buf = ctypes.create_string_buffer(bytes(stream), len(stream))
+        if self.fns["replay"](self.handle, buf, len(stream)) != 0:
+            raise ValueError("malformed command stream")
+
+    def get_image_data(self, width=None, height=None, stride=None, x=0, y=0, out=None):
+        width = self.width if width is None else width
+        height = self.height if height is None else height
+        stride = 4 * width if stride is None else stride
+        if out is None:
+            out = np.zeros((height, stride), dtype=np.uint8)
+        self.fns["get_image_data"](self.handle, out.ctypes.data_as(_P), width, height,
+                                   stride, x, y)
+        return out
+
+    def rgba8(self):
+        return self.get_image_data().reshape(self.height, self.width, 4)
+
+    def linear(self):
+        out = np.zeros((self.height, self.width, 4), dtype=np.float32)
+        self.fns["read_linear"](self.handle, out.ctypes.data_as(_P))
+        return out
+
+    def path_lines(self, stroking):
+        """Reference seam: lines after path_to_lines (+ stroke_lines)."""
+        lib = self.fns["lib"]
+        cap = 1 << 22
+        pts = np.zeros((cap, 2), dtype=np.float32)
+        sub = np.zeros((cap // 2, 2), dtype=np.int32)
+        ns = ctypes.c_long()
+        n = lib.cih_ref_path_lines(self.handle, int(stroking), pts.ctypes.data_as(_P), cap,
+                                   sub.ctypes.data_as(_P), cap // 2, ctypes.byref(ns))
+        return pts[:n].copy(), sub[:ns.value].copy()
+
+    def lines_to_runs(self, offset_x=0.0, offset_y=0.0, padding=0):
+        """Reference seam: sorted, coalesced runs of the current lines."""
+        lib = self.fns["lib"]
+        cap = 1 << 23
+        runs = np.zeros((cap, 3), dtype=np.float32)
+        n = lib.cih_ref_lines_to_runs(self.handle, offset_x, offset_y, padding,
+                                      runs.ctypes.data_as(_P), cap)
+        return runs[:n].copy()
+
+    def close(self):
+        if self.handle:
+            self.fns["destroy"](self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def render(stream, kind="port"):
+    """Replays a command stream on a fresh CPU canvas; returns (sRGB8, linear)."""
+    words = np.frombuffer(stream[:16], dtype=np.uint32)
+    canvas = Canvas(int(words[2]), int(words[3]), kind)
+    try:
+        canvas.replay(stream)
+        return canvas.rgba8(), canvas.linear()
+    finally:
+        canvas.close()

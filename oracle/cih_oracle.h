@@ -1,0 +1,37 @@
+/*
+ * cih_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * The CPU oracle: a plain-C restatement of canvas_ity's rasterise-and-
+ * composite pipeline (canvas_ity.hpp), used by the test suite, smoke() and
+ * bench.py's CPU baseline as the checker of the HIP library.  The product
+ * never links it.  Every function in cih_oracle.c cites the reference lines it
+ * follows.  How it is pinned (reference build, known-answer vectors, closed
+ * forms) is recorded in tests/test_oracle.py.
+ */
+#ifndef CIH_ORACLE_H
+#define CIH_ORACLE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct cih_oracle cih_oracle;
+
+cih_oracle *cih_oracle_create(int width, int height);
+void cih_oracle_destroy(cih_oracle *canvas);
+/* Replays a command stream (include/cih_stream.h); 0 or -1 if malformed. */
+int cih_oracle_replay(cih_oracle *canvas, const void *stream, size_t bytes);
+void cih_oracle_get_image_data(cih_oracle *canvas, unsigned char *image, int width,
+                               int height, int stride, int x, int y);
+void cih_oracle_put_image_data(cih_oracle *canvas, const unsigned char *image, int width,
+                               int height, int stride, int x, int y);
+/* The fp32 linear premultiplied RGBA bitmap. */
+void cih_oracle_read_linear(cih_oracle *canvas, float *rgba);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CIH_ORACLE_H */
