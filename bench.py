@@ -1,0 +1,159 @@
+"""Benchmark of the rasterise-and-composite hot path on one MI355X.
+
+Workload (one "step"): the large-2048 configuration -- a 2048x2048 canvas with
+5000 seeded shapes (60% fills / 40% strokes; cubic and quadratic paths, arcs,
+rectangles; solid, linear and radial brushes with 2-8 stops; dashes; all
+joins and caps; global alpha) -- rendered from scratch by the plain
+per-canvas GPU path: clear, flatten/dash/stroke, scan conversion, shading and
+compositing of every draw, and the sRGB8 conversion of the whole canvas.  The
+draw packets are resident in HBM before the timed region (the host-side
+command replay and the PCIe transfers are outside it); value = canvas pixels
+per second.
+
+The JSON line also carries the roofline of the dominant kernel (its time
+from HIP events recorded by the library on its own stream during the timed
+steps) and a CPU baseline: the reference canvas_ity compiled from source
+(oracle/_ref) timed on this host on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mpix/s + canvases/s (batched vs plain GPU), p99 latency at fixed QPS, HBM GB/s"
+SEED = 0xC2
+
+
+def cpu_baseline(stream, width, height, sample_draws, threads):
+    """Times the CPU reference on the first `sample_draws` draws of the stream
+    (single thread, then one instance per thread) and scales to one canvas."""
+    import oracle
+    from canvas_ity_amd.stream import count_draws, truncate_draws
+    kind = "reference" if oracle.available("reference") else "port"
+    total = count_draws(stream)
+    sample = truncate_draws(stream, sample_draws)
+    scale = total / float(sample_draws)
+    t0 = time.perf_counter()
+    oracle.render(sample, kind)
+    single = time.perf_counter() - t0
+    per_canvas = single * scale
+    result = {"value": width * height / 1e6 / per_canvas, "unit": "Mpix/s", "cores": 1,
+              "kind": kind,
+              "sample": "first %d of %d draws of the same 2048x2048 stream, one thread, "
+                        "scaled by %.3g to a whole canvas (%.1f s measured)"
+                        % (sample_draws, total, scale, single)}
+    if threads > 1:
+        times = [0.0] * threads
+
+        def work(i):
+            t = time.perf_counter()
+            oracle.render(sample, kind)
+            times[i] = time.perf_counter() - t
+        workers = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+        t0 = time.perf_counter()
+        for w in workers:
+            w.start()
+        for w in workers:
+            w.join()
+        wall = time.perf_counter() - t0
+        result["per_core"] = {"value": threads * width * height / 1e6 / (wall * scale),
+                              "unit": "Mpix/s", "cores": threads,
+                              "sample": "%d concurrent instances of the sample" % threads}
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=1500,
+                    help="draws of the workload timed on the CPU reference (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="instances for the per-core CPU figure (0: min(16, cores))")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build renders independent canvases per GPU "
+              "and has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import canvas_ity_amd as cia
+    stream = cia.scene(2, SEED)
+    width, height = cia.stream_size(stream)
+    device = cia.default_device()
+    canvas = cia.Canvas(width, height, device)
+    canvas.replay(stream)  # host state machine -> draw packets
+    canvas.flush()  # packets and decoded draws now resident in HBM
+
+    for _ in range(args.warmup):
+        canvas.execute(clear=True, readback=True)
+    device.synchronize()
+
+    device.profile(True)
+    device.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        canvas.execute(clear=True, readback=True)
+    device.synchronize()
+    t1 = time.perf_counter()
+    prof = device.profile_read()
+    device.profile(False)
+
+    ms = (t1 - t0) * 1000.0 / args.steps
+    mpix = width * height / 1e6 / (ms / 1000.0)
+    frames = max(prof.pop("frames"), 1)
+    kernels = {k: v / frames for k, v in prof.items()}  # ms per launch
+    dominant = max(kernels, key=kernels.get)
+
+    # Algorithmic bytes of the composite kernel: 32 B (fp32 RGBA read + write)
+    # per pixel-draw, counted exactly by the library's statistics pass.
+    stats = canvas_stats(cia, canvas, device)
+    pixel_draws = stats["pixel_draws"]
+    line = {
+        "metric": METRIC, "value": round(mpix, 3), "unit": "Mpix/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded command stream %#x)" % SEED,
+        "config": {"workload": "large-2048 plain path: 2048x2048 canvas, 5000 seeded shapes "
+                               "(fills/strokes, beziers/arcs/rects, gradients, dashes, joins), "
+                               "sRGB8 read-back", "canvas": [width, height], "draws": 5000,
+                   "path": "plain per-canvas GPU"},
+        "canvases_per_s": round(1000.0 / ms, 3),
+        "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
+    }
+    fine_ms = kernels.get("k_fine", 0.0)
+    achieved = 32.0 * pixel_draws / (fine_ms / 1000.0) / 1e9 if fine_ms > 0 else 0.0
+    line["roofline"] = {"kernel": "k_fine", "dominant_kernel": dominant, "bound": "hbm",
+                        "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(),
+                        "pixel_draws": pixel_draws}
+    if args.cpu_sample > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(stream, width, height, args.cpu_sample, threads)
+    print(json.dumps(line))
+    return 0
+
+
+def canvas_stats(cia, canvas, device):
+    return canvas.stats()
+
+
+def traffic_bytes():
+    """HBM bytes per k_fine launch from the committed rocprofv3 counter
+    summary (profiles/k_fine_large2048_pmc.json), or None."""
+    path = os.path.join(ROOT, "profiles", "k_fine_large2048_pmc.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+if __name__ == "__main__":
+    sys.exit(main())

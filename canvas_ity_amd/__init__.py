@@ -45,10 +45,22 @@ SIGNATURES = {
                                              _c.c_int, _c.c_int]),
     "cih_canvas_read_linear": (_c.c_int, [_P, _P]),
     "cih_canvas_replay": (_c.c_int, [_P, _P, _c.c_size_t]),
+    "cih_canvas_execute": (_c.c_int, [_P, _c.c_int, _c.c_int]),
+    "cih_canvas_stats": (_c.c_int, [_P, _c.POINTER(_c.c_uint64)]),
+    "cih_device_profile": (_c.c_int, [_P, _c.c_int]),
+    "cih_device_profile_read": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
     "cih_trace_polygons": (_c.c_int, [_P, _c.c_int, _P, _c.c_size_t, _P, _c.c_size_t,
                                       _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)]),
     "cih_trace_runs": (_c.c_int, [_P, _c.c_int, _c.c_int, _P, _P, _P, _c.c_size_t,
                                   _c.POINTER(_c.c_size_t)]),
+    "cih_batch_create": (_c.c_int, [_P, _c.c_int, _c.POINTER(_P), _c.POINTER(_c.c_size_t),
+                                    _c.POINTER(_P)]),
+    "cih_batch_execute": (_c.c_int, [_P, _c.c_int]),
+    "cih_batch_size": (_c.c_int, [_P, _c.c_int, _c.POINTER(_c.c_int), _c.POINTER(_c.c_int)]),
+    "cih_batch_image": (_c.c_int, [_P, _c.c_int, _P]),
+    "cih_batch_linear": (_c.c_int, [_P, _c.c_int, _P]),
+    "cih_batch_stats": (_c.c_int, [_P, _c.POINTER(_c.c_uint64)]),
+    "cih_batch_destroy": (_c.c_int, [_P]),
     "cih_scene_generate": (_c.c_int, [_c.c_int, _c.c_uint64, _c.c_int, _c.c_int, _c.c_int,
                                       _c.c_int, _c.POINTER(_P), _c.POINTER(_c.c_size_t)]),
     "cih_scene_seed": (_c.c_uint64, [_c.c_uint64, _c.c_uint64]),
@@ -117,6 +129,16 @@ class Device:
     def synchronize(self):
         _check(library().cih_device_synchronize(self.handle), "cih_device_synchronize")
 
+    def profile(self, enable=True):
+        _check(library().cih_device_profile(self.handle, int(enable)), "cih_device_profile")
+
+    def profile_read(self):
+        import json
+        buf = _c.create_string_buffer(4096)
+        _check(library().cih_device_profile_read(self.handle, buf, 4096),
+               "cih_device_profile_read")
+        return json.loads(buf.value.decode())
+
     def close(self):
         if self.handle:
             library().cih_device_close(self.handle)
@@ -149,6 +171,18 @@ class Canvas:
 
     def flush(self):
         _check(library().cih_canvas_flush(self.handle), "cih_canvas_flush")
+
+    def stats(self):
+        """Work counts of the last draw list (runs it once, synchronously)."""
+        out = (_c.c_uint64 * 4)()
+        _check(library().cih_canvas_stats(self.handle, out), "cih_canvas_stats")
+        return {"pixel_draws": out[0], "runs": out[1], "points": out[2],
+                "shadow_floats": out[3]}
+
+    def execute(self, clear=True, readback=True):
+        """Runs the last flushed draw list again from HBM (asynchronous)."""
+        _check(library().cih_canvas_execute(self.handle, int(clear), int(readback)),
+               "cih_canvas_execute")
 
     def get_image_data(self, width=None, height=None, stride=None, x=0, y=0, out=None):
         width = self.width if width is None else width
@@ -207,6 +241,59 @@ class Canvas:
     def close(self):
         if self.handle:
             library().cih_canvas_destroy(self.handle)
+            self.handle = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many canvases rendered together (cih_batch_* in include/cih_abi.h)."""
+
+    def __init__(self, streams, device=None):
+        self.device = device or default_device()
+        self._keep = [_c.create_string_buffer(bytes(s), len(s)) for s in streams]
+        n = len(streams)
+        ptrs = (_P * n)(*[_c.cast(b, _P) for b in self._keep])
+        sizes = (_c.c_size_t * n)(*[len(s) for s in streams])
+        self.handle = _P()
+        _check(library().cih_batch_create(self.device.handle, n, ptrs, sizes,
+                                          _c.byref(self.handle)), "cih_batch_create")
+        self.count = n
+        self.sizes = []
+        for i in range(n):
+            w, h = _c.c_int(), _c.c_int()
+            library().cih_batch_size(self.handle, i, _c.byref(w), _c.byref(h))
+            self.sizes.append((w.value, h.value))
+
+    def execute(self, to_host=True):
+        _check(library().cih_batch_execute(self.handle, int(to_host)), "cih_batch_execute")
+
+    def rgba8(self, index):
+        w, h = self.sizes[index]
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        _check(library().cih_batch_image(self.handle, index, out.ctypes.data_as(_P)),
+               "cih_batch_image")
+        return out
+
+    def linear(self, index):
+        w, h = self.sizes[index]
+        out = np.zeros((h, w, 4), dtype=np.float32)
+        _check(library().cih_batch_linear(self.handle, index, out.ctypes.data_as(_P)),
+               "cih_batch_linear")
+        return out
+
+    def stats(self):
+        out = (_c.c_uint64 * 4)()
+        _check(library().cih_batch_stats(self.handle, out), "cih_batch_stats")
+        return {"canvases": out[0], "draws": out[1], "frames": out[2], "pixels": out[3]}
+
+    def close(self):
+        if self.handle:
+            library().cih_batch_destroy(self.handle)
             self.handle = _P()
 
     def __del__(self):

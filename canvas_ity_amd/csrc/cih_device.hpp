@@ -70,6 +70,7 @@ enum counter {
     COUNTER_SHADOW = 3,
     COUNTER_OVERFLOW = 4,
     COUNTER_QUEUE = 5,
+    COUNTER_PIXEL_DRAWS = 6, // pixels composited by k_fine (main and shadow layers)
     COUNTER_COUNT = 8
 };
 

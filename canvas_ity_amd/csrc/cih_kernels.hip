@@ -1049,6 +1049,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
         back[k] = xs[k] < C.width ? bitmap[xs[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     __syncthreads();
+    unsigned composites = 0;
     for (int d = C.first_draw; d < C.first_draw + C.ndraws; ++d) {
         const draw_desc &D = F.draws[d];
         const draw_state &S = F.state[d];
@@ -1068,6 +1069,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
                         continue;
                     float s = row[x + D.border - S.left];
                     composite(back[k], scaled(color, D.global_alpha * s), op, 1.0f);
+                    ++composites;
                 }
             }
         }
@@ -1094,9 +1096,15 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
             if (cov >= kThreshold || clear) {
                 float4 p = paint(F, D, mk((float)x + 0.5f, (float)y + 0.5f));
                 composite(back[k], scaled(p, cov * D.global_alpha), op, 1.0f);
+                ++composites;
             }
         }
     }
+    unsigned long long total = composites;
+    for (int w = 32; w >= 1; w >>= 1)
+        total += __shfl_xor(total, w, 64);
+    if ((threadIdx.x & 63) == 0 && total)
+        atomicAdd(&F.counters[COUNTER_PIXEL_DRAWS], total);
     for (int k = 0; k < FINE_PX; ++k)
         if (xs[k] < C.width)
             bitmap[xs[k]] = back[k];
@@ -1185,21 +1193,44 @@ namespace cih {
 
 int fine_segment() { return FINE_SEGMENT; }
 
+const int kFrameKernels = 7;
+extern const char *const kFrameKernelNames[kFrameKernels] = {
+    "k_reset", "k_geometry", "k_runs", "k_shadow_raster", "k_blur_rows", "k_blur_cols",
+    "k_fine"};
+
+// Launches a frame's kernels on `stream`.  With `events` (kFrameKernels + 1
+// of them), event k is recorded before kernel k and the last after the last
+// kernel, so the time of kernel k is elapsed(events[k], events[k + 1]).
 hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool any_shadow,
-                        hipStream_t stream) {
+                        hipEvent_t *events, hipStream_t stream) {
+#define MARK(k)                                                                                \
+    do {                                                                                       \
+        if (events)                                                                            \
+            hipEventRecord(events[k], stream);                                                 \
+    } while (0)
+    MARK(0);
     k_reset<<<1, 64, 0, stream>>>(F.counters);
-    if (F.ndraws > 0) {
+    MARK(1);
+    if (F.ndraws > 0)
         k_geometry<<<F.ndraws, GEO_THREADS, 0, stream>>>(F);
+    MARK(2);
+    if (F.ndraws > 0)
         k_runs<<<2 * F.ndraws, GEO_THREADS, 0, stream>>>(F);
-        if (any_shadow) {
-            k_shadow_raster<<<dim3(F.ndraws, 64), FINE_THREADS, 0, stream>>>(F);
-            int ry = (shadow_rows + 63) / 64, cx = (shadow_cols + 63) / 64;
-            k_blur_rows<<<dim3(F.ndraws, ry > 0 ? ry : 1), 64, 0, stream>>>(F);
-            k_blur_cols<<<dim3(F.ndraws, cx > 0 ? cx : 1), 64, 0, stream>>>(F);
-        }
-    }
+    MARK(3);
+    int ry = (shadow_rows + 63) / 64, cx = (shadow_cols + 63) / 64;
+    if (F.ndraws > 0 && any_shadow)
+        k_shadow_raster<<<dim3(F.ndraws, 64), FINE_THREADS, 0, stream>>>(F);
+    MARK(4);
+    if (F.ndraws > 0 && any_shadow)
+        k_blur_rows<<<dim3(F.ndraws, ry > 0 ? ry : 1), 64, 0, stream>>>(F);
+    MARK(5);
+    if (F.ndraws > 0 && any_shadow)
+        k_blur_cols<<<dim3(F.ndraws, cx > 0 ? cx : 1), 64, 0, stream>>>(F);
+    MARK(6);
     if (F.nitems)
         k_fine<<<F.nitems, FINE_THREADS, 0, stream>>>(F);
+    MARK(7);
+#undef MARK
     return hipGetLastError();
 }
 
@@ -1231,6 +1262,77 @@ hipError_t launch_readback(const void *bitmap, int W, int H, void *out, int w, i
                            int y, hipStream_t stream) {
     k_readback<<<grid_for((size_t)w * h, 256), 256, 0, stream>>>(
         (const float4 *)bitmap, W, H, (uint32_t *)out, w, h, x, y);
+    return hipGetLastError();
+}
+
+} // namespace cih
+
+// ---------------------------------------------------------------------------
+// Batched read-back and clear: every canvas of a batch, one launch each.
+
+namespace cih {
+
+__device__ inline int canvas_of(const uint64_t *prefix, int n, uint64_t i) {
+    int lo = 0, hi = n; // prefix[lo] <= i < prefix[lo + 1]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_readback_batch(const canvas_desc *canvases, const uint64_t *pixel_prefix,
+                                 int ncanvases, uint32_t *out) {
+    uint64_t total = pixel_prefix[ncanvases];
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        int k = canvas_of(pixel_prefix, ncanvases, i);
+        const canvas_desc &C = canvases[k];
+        uint64_t local = i - pixel_prefix[k];
+        int cx = (int)(local % (uint64_t)C.width), cy = (int)(local / (uint64_t)C.width);
+        float4 c = ((const float4 *)(uintptr_t)C.bitmap)[local];
+        float t = c_bayer[(cy & 3) * 4 + (cx & 3)];
+        if (c.w < 1.0f / 8160.0f)
+            c = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        else {
+            float inv = 1.0f / c.w;
+            c = make_float4(inv * c.x, inv * c.y, inv * c.z, c.w);
+        }
+        float r = t + delinearized(clamp01(c.x)) * 255.0f;
+        float g = t + delinearized(clamp01(c.y)) * 255.0f;
+        float b = t + delinearized(clamp01(c.z)) * 255.0f;
+        float a = t + clamp01(c.w) * 255.0f;
+        out[i] = (uint32_t)(unsigned char)(int)r | ((uint32_t)(unsigned char)(int)g << 8) |
+                 ((uint32_t)(unsigned char)(int)b << 16) | ((uint32_t)(unsigned char)(int)a << 24);
+    }
+}
+
+__global__ void k_clear_batch(const canvas_desc *canvases, const uint64_t *pixel_prefix,
+                              int ncanvases) {
+    uint64_t total = pixel_prefix[ncanvases];
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        int k = canvas_of(pixel_prefix, ncanvases, i);
+        ((float4 *)(uintptr_t)canvases[k].bitmap)[i - pixel_prefix[k]] =
+            make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+hipError_t launch_readback_batch(const void *canvases, const void *pixel_prefix, int ncanvases,
+                                 uint64_t total, void *out, hipStream_t stream) {
+    k_readback_batch<<<grid_for(total, 256), 256, 0, stream>>>(
+        (const canvas_desc *)canvases, (const uint64_t *)pixel_prefix, ncanvases,
+        (uint32_t *)out);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_batch(const void *canvases, const void *pixel_prefix, int ncanvases,
+                              uint64_t total, hipStream_t stream) {
+    k_clear_batch<<<grid_for(total, 256), 256, 0, stream>>>(
+        (const canvas_desc *)canvases, (const uint64_t *)pixel_prefix, ncanvases);
     return hipGetLastError();
 }
 

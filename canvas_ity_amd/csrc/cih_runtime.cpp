@@ -12,6 +12,7 @@
 
 #include "cih_abi.h"
 #include "cih_device.hpp"
+#include "cih_stream.h"
 
 #include <algorithm>
 #include <cmath>
@@ -20,14 +21,22 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <system_error>
+#include <thread>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 namespace cih {
 int fine_segment();
+hipError_t launch_readback_batch(const void *canvases, const void *pixel_prefix, int ncanvases,
+                                 uint64_t total, void *out, hipStream_t stream);
+hipError_t launch_clear_batch(const void *canvases, const void *pixel_prefix, int ncanvases,
+                              uint64_t total, hipStream_t stream);
+const int kFrameKernels = 7; // reset, geometry, runs, shadow raster, blur rows/cols, fine
+extern const char *const kFrameKernelNames[kFrameKernels];
 hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool any_shadow,
-                        hipStream_t stream);
+                        hipEvent_t *events, hipStream_t stream);
 hipError_t launch_clear(void *pixels, size_t n, hipStream_t stream);
 hipError_t launch_prepare_image(const void *rgba8, void *texels, size_t n, const float *lut,
                                 hipStream_t stream);
@@ -150,7 +159,7 @@ struct cih_device {
     // arenas
     dbuf points, words, runs, shadow, counters;
     // frame inputs
-    dbuf payload, draws, state, canvases, items;
+    dbuf state, canvases, items;
     hbuf staging, host_counters;
     // images
     std::vector<image_entry *> images;
@@ -162,13 +171,31 @@ struct cih_device {
     hbuf readback_host;
     size_t frames = 0, retries = 0;
     size_t last_ndraws = 0; // draws of the last frame (for the trace seams)
+    // kernel timing with HIP events on the device's stream
+    bool profiling = false;
+    bool capturing = false; // a HIP graph capture is recording on the stream
+    std::vector<hipEvent_t> event_pool;
+    size_t events_used = 0;
+};
+
+// A canvas' decoded draw list, resident in HBM so it can be run again.
+struct program {
+    dbuf payload, draws;
+    std::vector<draw_desc> descs; // host copy
+    struct span {
+        size_t first, count;
+    };
+    std::vector<span> frames;
+    size_t bytes = 0;
 };
 
 struct cih_canvas {
     cih_device *device;
     int width, height;
+    void *pixels = nullptr; // float4 bitmap in HBM (owned by `bitmap` or by a batch)
     dbuf bitmap;
     std::vector<unsigned char> pending;
+    program program;
 };
 
 namespace {
@@ -278,79 +305,95 @@ int ensure_arenas(cih_device *d, uint64_t pts, uint64_t words, uint64_t runs, ui
     return CIH_OK;
 }
 
-// One canvas' slice of a frame.
-struct frame_canvas {
-    cih_canvas *canvas;
-    size_t first, count; // draws in the frame's desc list
+// Hands out kFrameKernels + 1 events for one profiled frame.
+int profile_events(cih_device *d, hipEvent_t **events) {
+    size_t need = d->events_used + kFrameKernels + 1;
+    while (d->event_pool.size() < need) {
+        hipEvent_t e;
+        HIP_TRY(hipEventCreate(&e));
+        d->event_pool.push_back(e);
+    }
+    *events = &d->event_pool[d->events_used];
+    d->events_used = need;
+    return CIH_OK;
+}
+
+// A frame: draws (already in HBM) of one or more canvases, run together.
+struct frame_input {
+    const unsigned char *payload; // device
+    const draw_desc *draws;       // device, n records
+    const draw_desc *host;        // host copy of the same records
+    size_t n;
+    std::vector<canvas_desc> canvases; // first_draw relative to `draws`
 };
 
-// Runs one frame (already decoded draws of one or more canvases, payload
-// uploaded) to completion, growing arenas and retrying on overflow.
-int run_frame(cih_device *d, std::vector<draw_desc> &descs,
-              const std::vector<frame_canvas> &slices, bool sync) {
-    int s;
-    size_t n = descs.size();
-    std::vector<canvas_desc> canv(slices.size());
-    std::vector<uint32_t> items(slices.size() + 1, 0);
-    int seg = fine_segment();
-    int max_rows = 0, max_cols = 0;
+// What a frame needs from the arenas, and its fine-stage work items.
+struct frame_shape {
     bool any_shadow = false;
-    uint64_t shadow_bound = 0;
-    uint64_t in_points = 0;
-    for (size_t c = 0; c < slices.size(); ++c) {
-        cih_canvas *cv = slices[c].canvas;
-        canv[c].bitmap = (uint64_t)(uintptr_t)cv->bitmap.ptr;
-        canv[c].width = cv->width;
-        canv[c].height = cv->height;
-        canv[c].first_draw = (int32_t)slices[c].first;
-        canv[c].ndraws = (int32_t)slices[c].count;
-        uint64_t segs = (uint64_t)((cv->width + seg - 1) / seg);
-        items[c + 1] = items[c] + (uint32_t)(segs * (uint64_t)cv->height);
-        for (size_t k = slices[c].first; k < slices[c].first + slices[c].count; ++k) {
-            descs[k].canvas = (int32_t)c;
-            in_points += descs[k].npoints;
-            if (descs[k].shadow) {
-                any_shadow = true;
-                int b = descs[k].border;
-                max_rows = std::max(max_rows, cv->height + 2 * b + 1);
-                max_cols = std::max(max_cols, cv->width + 2 * b + 1);
-                shadow_bound += 2ull * (uint64_t)(cv->width + 2 * b + 1) *
-                                (uint64_t)(cv->height + 2 * b + 1);
+    int max_rows = 0, max_cols = 0;
+    uint64_t shadow_bound = 0, in_points = 0;
+    size_t ndraws = 0;
+    std::vector<uint32_t> items; // per-canvas prefix of (row, segment) items
+};
+
+frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t ncanvases) {
+    frame_shape f;
+    int seg = fine_segment();
+    f.items.assign(ncanvases + 1, 0);
+    for (size_t c = 0; c < ncanvases; ++c) {
+        const canvas_desc &cd = canvases[c];
+        uint64_t segs = (uint64_t)((cd.width + seg - 1) / seg);
+        f.items[c + 1] = f.items[c] + (uint32_t)(segs * (uint64_t)cd.height);
+        for (int k = cd.first_draw; k < cd.first_draw + cd.ndraws; ++k) {
+            const draw_desc &dd = host[k];
+            f.in_points += dd.npoints;
+            f.ndraws++;
+            if (dd.shadow) {
+                f.any_shadow = true;
+                int b = dd.border;
+                f.max_rows = std::max(f.max_rows, cd.height + 2 * b + 1);
+                f.max_cols = std::max(f.max_cols, cd.width + 2 * b + 1);
+                f.shadow_bound += 2ull * (uint64_t)(cd.width + 2 * b + 1) *
+                                  (uint64_t)(cd.height + 2 * b + 1);
             }
         }
     }
-    if ((s = d->draws.reserve(sizeof(draw_desc) * std::max<size_t>(n, 1))) ||
-        (s = d->state.reserve(sizeof(draw_state) * std::max<size_t>(n, 1))) ||
-        (s = d->canvases.reserve(sizeof(canvas_desc) * canv.size())) ||
-        (s = d->items.reserve(4 * items.size())) || (s = d->counters.reserve(64)) ||
-        (s = d->host_counters.reserve(64)))
-        return s;
-    // first-guess arena sizes; overflow grows them to what the frame asked for
-    uint64_t want_points = std::max<uint64_t>(1 << 22, 64 * in_points);
-    uint64_t want_words = std::max<uint64_t>(1 << 22, 8 * in_points + 64 * n);
-    uint64_t want_runs = std::max<uint64_t>(1 << 22, 16 * in_points);
-    if ((s = ensure_arenas(d, std::max<uint64_t>(want_points, d->points.bytes / 8),
-                           std::max<uint64_t>(want_words, d->words.bytes / 4),
-                           std::max<uint64_t>(want_runs, d->runs.bytes / sizeof(run)),
-                           std::max<uint64_t>(shadow_bound, d->shadow.bytes / 4))))
-        return s;
-    HIP_TRY(hipMemcpyAsync(d->draws.ptr, descs.data(), sizeof(draw_desc) * n,
-                           hipMemcpyHostToDevice, d->stream));
-    HIP_TRY(hipMemcpyAsync(d->canvases.ptr, canv.data(), sizeof(canvas_desc) * canv.size(),
-                           hipMemcpyHostToDevice, d->stream));
-    HIP_TRY(hipMemcpyAsync(d->items.ptr, items.data(), 4 * items.size(), hipMemcpyHostToDevice,
-                           d->stream));
+    return f;
+}
+
+// Launches the kernels of one frame whose canvas list and work items are
+// already in HBM.  With `check`, waits for it and, if an arena ran out, grows
+// the arenas to what the frame asked for and runs it again (kernels write no
+// pixel once an arena has overflowed).  Without it nothing synchronises or
+// allocates (arenas must already fit: graph capture relies on that).
+int run_kernels(cih_device *d, const unsigned char *payload, const draw_desc *draws, size_t n,
+                const canvas_desc *canvases_dev, size_t ncanvases, const uint32_t *items_dev,
+                const frame_shape &shape, bool check) {
+    int s;
+    if (check) {
+        if ((s = d->state.reserve(sizeof(draw_state) * std::max<size_t>(n, 1))) ||
+            (s = d->counters.reserve(64)) || (s = d->host_counters.reserve(64)))
+            return s;
+        uint64_t want_points = std::max<uint64_t>(1 << 22, 64 * shape.in_points);
+        uint64_t want_words = std::max<uint64_t>(1 << 22, 8 * shape.in_points + 64 * n);
+        uint64_t want_runs = std::max<uint64_t>(1 << 22, 16 * shape.in_points);
+        if ((s = ensure_arenas(d, std::max<uint64_t>(want_points, d->points.bytes / 8),
+                               std::max<uint64_t>(want_words, d->words.bytes / 4),
+                               std::max<uint64_t>(want_runs, d->runs.bytes / sizeof(run)),
+                               std::max<uint64_t>(shape.shadow_bound, d->shadow.bytes / 4))))
+            return s;
+    }
     d->last_ndraws = n;
     for (int attempt = 0; attempt < 8; ++attempt) {
         frame F;
         std::memset(&F, 0, sizeof(F));
-        F.payload = (const unsigned char *)d->payload.ptr;
-        F.draws = (const draw_desc *)d->draws.ptr;
+        F.payload = payload;
+        F.draws = draws;
         F.state = (draw_state *)d->state.ptr;
-        F.canvases = (const canvas_desc *)d->canvases.ptr;
+        F.canvases = canvases_dev;
         F.images = (const image_desc *)d->image_table.ptr;
         F.ndraws = (int32_t)n;
-        F.ncanvases = (int32_t)canv.size();
+        F.ncanvases = (int32_t)ncanvases;
         F.points = (float *)d->points.ptr;
         F.points_cap = d->points.bytes / 8;
         F.words = (uint32_t *)d->words.ptr;
@@ -360,13 +403,21 @@ int run_frame(cih_device *d, std::vector<draw_desc> &descs,
         F.shadow = (float *)d->shadow.ptr;
         F.shadow_cap = d->shadow.bytes / 4;
         F.counters = (unsigned long long *)d->counters.ptr;
-        F.items = (const uint32_t *)d->items.ptr;
-        F.nitems = items.back();
-        F.segment = seg;
-        HIP_TRY(launch_frame(F, max_rows, max_cols, any_shadow, d->stream));
+        F.items = items_dev;
+        F.nitems = shape.items.back();
+        F.segment = fine_segment();
+        hipEvent_t *events = nullptr;
+        if (d->profiling && !d->capturing) {
+            if ((s = profile_events(d, &events)))
+                return s;
+        }
+        HIP_TRY(launch_frame(F, shape.max_rows, shape.max_cols, shape.any_shadow, events,
+                             d->stream));
         d->frames++;
-        HIP_TRY(hipMemcpyAsync(d->host_counters.ptr, d->counters.ptr,
-                               8 * COUNTER_COUNT, hipMemcpyDeviceToHost, d->stream));
+        if (!check)
+            return CIH_OK;
+        HIP_TRY(hipMemcpyAsync(d->host_counters.ptr, d->counters.ptr, 8 * COUNTER_COUNT,
+                               hipMemcpyDeviceToHost, d->stream));
         HIP_TRY(hipStreamSynchronize(d->stream));
         const unsigned long long *hc = (const unsigned long long *)d->host_counters.ptr;
         if (!hc[COUNTER_OVERFLOW])
@@ -376,7 +427,6 @@ int run_frame(cih_device *d, std::vector<draw_desc> &descs,
         uint64_t nw = std::max<uint64_t>(hc[COUNTER_WORDS], F.words_cap);
         uint64_t nr = std::max<uint64_t>(hc[COUNTER_RUNS], F.runs_cap);
         uint64_t ns = std::max<uint64_t>(hc[COUNTER_SHADOW], F.shadow_cap);
-        // whichever arena ran out asked for more than it has; grow all that did
         if ((s = ensure_arenas(d, hc[COUNTER_POINTS] >= F.points_cap ? 2 * np : np,
                                hc[COUNTER_WORDS] >= F.words_cap ? 2 * nw : nw,
                                hc[COUNTER_RUNS] >= F.runs_cap ? 2 * nr : nr,
@@ -386,15 +436,32 @@ int run_frame(cih_device *d, std::vector<draw_desc> &descs,
     return fail(CIH_ERROR_OUT_OF_MEMORY, "frame did not fit after repeated arena growth");
 }
 
-// Decodes and runs all pending packets of one canvas.
-int flush_canvas(cih_canvas *cv) {
+// Launches one frame of a single canvas (uploads its small canvas list).
+int launch(cih_device *d, const frame_input &in, bool check) {
+    int s;
+    frame_shape shape = shape_of(in.host, in.canvases.data(), in.canvases.size());
+    if ((s = d->canvases.reserve(sizeof(canvas_desc) * in.canvases.size())) ||
+        (s = d->items.reserve(4 * shape.items.size())))
+        return s;
+    HIP_TRY(hipMemcpyAsync(d->canvases.ptr, in.canvases.data(),
+                           sizeof(canvas_desc) * in.canvases.size(), hipMemcpyHostToDevice,
+                           d->stream));
+    HIP_TRY(hipMemcpyAsync(d->items.ptr, shape.items.data(), 4 * shape.items.size(),
+                           hipMemcpyHostToDevice, d->stream));
+    return run_kernels(d, in.payload, in.draws, in.n, (const canvas_desc *)d->canvases.ptr,
+                       in.canvases.size(), (const uint32_t *)d->items.ptr, shape, check);
+}
+
+// Decodes pending packets into the canvas' resident program (payload and
+// draw records in HBM) split into frames of bounded size.
+int prepare_program(cih_canvas *cv) {
     cih_device *d = cv->device;
-    if (cv->pending.empty())
-        return CIH_OK;
     int s;
     std::vector<unsigned char> bytes;
     bytes.swap(cv->pending);
-    std::vector<draw_desc> descs;
+    program &P = cv->program;
+    P.descs.clear();
+    P.frames.clear();
     size_t at = 0;
     while (at < bytes.size()) {
         draw_desc desc;
@@ -402,24 +469,31 @@ int flush_canvas(cih_canvas *cv) {
         if (!used)
             return fail(CIH_ERROR_MALFORMED, "malformed draw packet at byte " +
                                                  std::to_string(at));
-        descs.push_back(desc);
+        desc.canvas = 0;
+        P.descs.push_back(desc);
         at += used;
     }
-    if ((s = d->payload.reserve(bytes.size())) || (s = d->staging.reserve(bytes.size())))
+    if ((s = P.payload.reserve(std::max<size_t>(bytes.size(), 16))) ||
+        (s = P.draws.reserve(sizeof(draw_desc) * std::max<size_t>(P.descs.size(), 1))) ||
+        (s = d->staging.reserve(bytes.size() + sizeof(draw_desc) * P.descs.size())))
         return s;
-    std::memcpy(d->staging.ptr, bytes.data(), bytes.size());
-    HIP_TRY(hipMemcpyAsync(d->payload.ptr, d->staging.ptr, bytes.size(), hipMemcpyHostToDevice,
-                           d->stream));
+    // one pinned staging copy, two transfers
+    unsigned char *stage = (unsigned char *)d->staging.ptr;
+    std::memcpy(stage, bytes.data(), bytes.size());
+    std::memcpy(stage + bytes.size(), P.descs.data(), sizeof(draw_desc) * P.descs.size());
+    HIP_TRY(hipMemcpyAsync(P.payload.ptr, stage, bytes.size(), hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemcpyAsync(P.draws.ptr, stage + bytes.size(), sizeof(draw_desc) * P.descs.size(),
+                           hipMemcpyHostToDevice, d->stream));
     // Frames of bounded size: shadow images are the largest intermediates.
     const uint64_t shadow_budget = (uint64_t)1 << 30; // floats
     size_t first = 0;
-    while (first < descs.size()) {
+    while (first < P.descs.size()) {
         size_t last = first;
         uint64_t shadow = 0;
-        while (last < descs.size() && last - first < 8192) {
+        while (last < P.descs.size() && last - first < 8192) {
             uint64_t need = 0;
-            if (descs[last].shadow) {
-                int b = descs[last].border;
+            if (P.descs[last].shadow) {
+                int b = P.descs[last].border;
                 need = 2ull * (uint64_t)(cv->width + 2 * b + 1) * (uint64_t)(cv->height + 2 * b + 1);
             }
             if (last > first && shadow + need > shadow_budget)
@@ -427,17 +501,45 @@ int flush_canvas(cih_canvas *cv) {
             shadow += need;
             ++last;
         }
-        std::vector<draw_desc> chunk(descs.begin() + (ptrdiff_t)first,
-                                     descs.begin() + (ptrdiff_t)last);
-        std::vector<frame_canvas> slices(1);
-        slices[0].canvas = cv;
-        slices[0].first = 0;
-        slices[0].count = chunk.size();
-        if ((s = run_frame(d, chunk, slices, true)))
-            return s;
+        program::span sp = {first, last - first};
+        P.frames.push_back(sp);
         first = last;
     }
+    P.bytes = bytes.size();
     return CIH_OK;
+}
+
+// Runs the canvas' program (all its frames) in order.
+int run_program(cih_canvas *cv, bool check) {
+    program &P = cv->program;
+    int s;
+    for (const program::span &sp : P.frames) {
+        frame_input in;
+        in.payload = (const unsigned char *)P.payload.ptr;
+        in.draws = (const draw_desc *)P.draws.ptr + sp.first;
+        in.host = P.descs.data() + sp.first;
+        in.n = sp.count;
+        canvas_desc cd;
+        cd.bitmap = (uint64_t)(uintptr_t)cv->pixels;
+        cd.width = cv->width;
+        cd.height = cv->height;
+        cd.first_draw = 0;
+        cd.ndraws = (int32_t)sp.count;
+        in.canvases.push_back(cd);
+        if ((s = launch(cv->device, in, check)))
+            return s;
+    }
+    return CIH_OK;
+}
+
+// Decodes and runs all pending packets of one canvas.
+int flush_canvas(cih_canvas *cv) {
+    if (cv->pending.empty())
+        return CIH_OK;
+    int s;
+    if ((s = prepare_program(cv)))
+        return s;
+    return run_program(cv, true);
 }
 
 uint64_t fnv1a(const unsigned char *p, size_t n, uint64_t h) {
@@ -504,9 +606,10 @@ int cih_device_close(cih_device *d) {
         img->texels.release();
         delete img;
     }
-    dbuf *all[] = {&d->points, &d->words, &d->runs, &d->shadow, &d->counters, &d->payload,
-                   &d->draws, &d->state, &d->canvases, &d->items, &d->image_table, &d->lut,
-                   &d->upload, &d->readback};
+    dbuf *all[] = {&d->points, &d->words, &d->runs, &d->shadow, &d->counters, &d->state,
+                   &d->canvases, &d->items, &d->image_table, &d->lut, &d->upload, &d->readback};
+    for (hipEvent_t e : d->event_pool)
+        hipEventDestroy(e);
     for (dbuf *b : all)
         b->release();
     d->staging.release();
@@ -546,7 +649,8 @@ int cih_canvas_create(cih_device *d, int width, int height, cih_canvas **canvas)
         delete c;
         return s;
     }
-    hipError_t e = launch_clear(c->bitmap.ptr, n, d->stream);
+    c->pixels = c->bitmap.ptr;
+    hipError_t e = launch_clear(c->pixels, n, d->stream);
     if (e != hipSuccess) {
         c->bitmap.release();
         delete c;
@@ -564,6 +668,8 @@ int cih_canvas_destroy(cih_canvas *c) {
     set_device(d);
     hipStreamSynchronize(d->stream);
     c->bitmap.release();
+    c->program.payload.release();
+    c->program.draws.release();
     delete c;
     return CIH_OK;
 }
@@ -689,7 +795,7 @@ int cih_canvas_get_image_data(cih_canvas *c, unsigned char *image, int width, in
     size_t n = (size_t)width * (size_t)height;
     if ((s = d->readback.reserve(4 * n)) || (s = d->readback_host.reserve(4 * n)))
         return s;
-    HIP_TRY(launch_readback(c->bitmap.ptr, c->width, c->height, d->readback.ptr, width, height,
+    HIP_TRY(launch_readback(c->pixels, c->width, c->height, d->readback.ptr, width, height,
                             x, y, d->stream));
     HIP_TRY(hipMemcpyAsync(d->readback_host.ptr, d->readback.ptr, 4 * n, hipMemcpyDeviceToHost,
                            d->stream));
@@ -721,7 +827,7 @@ int cih_canvas_put_image_data(cih_canvas *c, const unsigned char *image, int wid
                     4 * (size_t)width);
     HIP_TRY(hipMemcpyAsync(d->upload.ptr, d->staging.ptr, 4 * n, hipMemcpyHostToDevice,
                            d->stream));
-    HIP_TRY(launch_put_image(d->upload.ptr, width, height, c->bitmap.ptr, c->width, c->height,
+    HIP_TRY(launch_put_image(d->upload.ptr, width, height, c->pixels, c->width, c->height,
                              x, y, (const float *)d->lut.ptr, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
     return CIH_OK;
@@ -735,12 +841,104 @@ int cih_canvas_read_linear(cih_canvas *c, float *rgba) {
     int s;
     if ((s = cih_canvas_flush(c)))
         return s;
-    HIP_TRY(hipMemcpyAsync(rgba, c->bitmap.ptr, 16 * (size_t)c->width * c->height,
+    HIP_TRY(hipMemcpyAsync(rgba, c->pixels, 16 * (size_t)c->width * c->height,
                            hipMemcpyDeviceToHost, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
     return CIH_OK;
 }
 
+
+int cih_canvas_execute(cih_canvas *c, int clear, int readback) {
+    if (!c)
+        return fail(CIH_ERROR_INVALID, "canvas is NULL");
+    cih_device *d = c->device;
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    int s;
+    if ((s = set_device(d)) || (s = cih_canvas_flush(c)))
+        return s;
+    size_t n = (size_t)c->width * (size_t)c->height;
+    if (clear)
+        HIP_TRY(launch_clear(c->pixels, n, d->stream));
+    if ((s = run_program(c, false)))
+        return s;
+    if (readback) {
+        if ((s = d->readback.reserve(4 * n)))
+            return s;
+        HIP_TRY(launch_readback(c->pixels, c->width, c->height, d->readback.ptr, c->width,
+                                c->height, 0, 0, d->stream));
+    }
+    return CIH_OK;
+}
+
+int cih_canvas_stats(cih_canvas *c, uint64_t *stats) {
+    if (!c || !stats)
+        return fail(CIH_ERROR_INVALID, "NULL argument");
+    cih_device *d = c->device;
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    int s;
+    if ((s = set_device(d)) || (s = cih_canvas_flush(c)))
+        return s;
+    for (int k = 0; k < 4; ++k)
+        stats[k] = 0;
+    HIP_TRY(launch_clear(c->pixels, (size_t)c->width * c->height, d->stream));
+    for (const program::span &sp : c->program.frames) {
+        frame_input in;
+        in.payload = (const unsigned char *)c->program.payload.ptr;
+        in.draws = (const draw_desc *)c->program.draws.ptr + sp.first;
+        in.host = c->program.descs.data() + sp.first;
+        in.n = sp.count;
+        canvas_desc cd;
+        cd.bitmap = (uint64_t)(uintptr_t)c->pixels;
+        cd.width = c->width;
+        cd.height = c->height;
+        cd.first_draw = 0;
+        cd.ndraws = (int32_t)sp.count;
+        in.canvases.push_back(cd);
+        if ((s = launch(d, in, true)))
+            return s;
+        const unsigned long long *hc = (const unsigned long long *)d->host_counters.ptr;
+        stats[0] += hc[COUNTER_PIXEL_DRAWS];
+        stats[1] += hc[COUNTER_RUNS];
+        stats[2] += hc[COUNTER_POINTS];
+        stats[3] += hc[COUNTER_SHADOW];
+    }
+    return CIH_OK;
+}
+
+int cih_device_profile(cih_device *d, int enable) {
+    if (!d)
+        return fail(CIH_ERROR_INVALID, "device is NULL");
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    d->profiling = enable != 0;
+    d->events_used = 0;
+    return CIH_OK;
+}
+
+int cih_device_profile_read(cih_device *d, char *json, size_t capacity) {
+    if (!d || !json || !capacity)
+        return fail(CIH_ERROR_INVALID, "invalid arguments");
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    double total[kFrameKernels] = {0};
+    size_t frames = d->events_used / (kFrameKernels + 1);
+    for (size_t f = 0; f < frames; ++f)
+        for (int k = 0; k < kFrameKernels; ++k) {
+            float ms = 0.0f;
+            hipEvent_t *e = &d->event_pool[f * (kFrameKernels + 1)];
+            HIP_TRY(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+            total[k] += ms;
+        }
+    std::string out = "{\"frames\": " + std::to_string(frames);
+    for (int k = 0; k < kFrameKernels; ++k) {
+        char buf[128];
+        std::snprintf(buf, sizeof(buf), ", \"%s\": %.6f", kFrameKernelNames[k], total[k]);
+        out += buf;
+    }
+    out += "}";
+    std::snprintf(json, capacity, "%s", out.c_str());
+    d->events_used = 0;
+    return CIH_OK;
+}
 
 int cih_trace_polygons(cih_canvas *c, int draw, float *points, size_t point_capacity,
                        uint32_t *subpaths, size_t subpath_capacity, size_t *npoints,
@@ -800,6 +998,353 @@ int cih_trace_runs(cih_canvas *c, int draw, int pass, int32_t *xs, int32_t *ys, 
             if (deltas)
                 deltas[k] = r[i].delta;
         }
+    return CIH_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batches: many independent canvases rendered by shared frames.
+
+struct cih_batch {
+    cih_device *device = nullptr;
+    std::vector<cih_canvas *> views;   // canvases over the shared bitmap storage
+    std::vector<uint64_t> pixel_prefix; // host, count + 1
+    dbuf bitmaps, all_canvases, prefix_dev, srgb;
+    hbuf srgb_host;
+    dbuf payload, draws;
+    std::vector<draw_desc> descs;
+    struct frame_rec {
+        size_t first_draw, ndraws, canvases_at, ncanvases, items_at;
+        frame_shape shape;
+    };
+    std::vector<frame_rec> frames;
+    dbuf frame_canvases, frame_items;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool ran = false;
+    bool use_graph = true;
+};
+
+namespace {
+
+// Replays streams [first, last) into their views (host work, any thread).
+void replay_range(cih_batch *b, const void *const *streams, const size_t *bytes, size_t first,
+                  size_t last, int *status) {
+    for (size_t i = first; i < last; ++i) {
+        int st = cih_canvas_replay(b->views[i], streams[i], bytes[i]);
+        if (st != CIH_OK) {
+            *status = st;
+            return;
+        }
+    }
+}
+
+void destroy_batch(cih_batch *b) {
+    if (b->exec)
+        hipGraphExecDestroy(b->exec);
+    if (b->graph)
+        hipGraphDestroy(b->graph);
+    for (cih_canvas *v : b->views)
+        delete v;
+    dbuf *all[] = {&b->bitmaps, &b->all_canvases, &b->prefix_dev, &b->srgb, &b->payload,
+                   &b->draws, &b->frame_canvases, &b->frame_items};
+    for (dbuf *x : all)
+        x->release();
+    b->srgb_host.release();
+    delete b;
+}
+
+// Records the whole batch (clear, frames, sRGB8 conversion) on the stream.
+int enqueue_batch(cih_batch *b, bool check) {
+    cih_device *d = b->device;
+    int s;
+    size_t n = b->views.size();
+    uint64_t total = b->pixel_prefix.back();
+    HIP_TRY(launch_clear_batch(b->all_canvases.ptr, b->prefix_dev.ptr, (int)n, total, d->stream));
+    for (const cih_batch::frame_rec &f : b->frames) {
+        if ((s = run_kernels(d, (const unsigned char *)b->payload.ptr,
+                             (const draw_desc *)b->draws.ptr + f.first_draw, f.ndraws,
+                             (const canvas_desc *)b->frame_canvases.ptr + f.canvases_at,
+                             f.ncanvases, (const uint32_t *)b->frame_items.ptr + f.items_at,
+                             f.shape, check)))
+            return s;
+    }
+    HIP_TRY(launch_readback_batch(b->all_canvases.ptr, b->prefix_dev.ptr, (int)n, total,
+                                  b->srgb.ptr, d->stream));
+    return CIH_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int cih_batch_create(cih_device *d, int count, const void *const *streams, const size_t *bytes,
+                     cih_batch **out) {
+    if (!d || count <= 0 || !streams || !bytes || !out)
+        return fail(CIH_ERROR_INVALID, "invalid batch arguments");
+    *out = nullptr;
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    int s;
+    if ((s = set_device(d)))
+        return s;
+    cih_batch *b = new (std::nothrow) cih_batch;
+    if (!b)
+        return fail(CIH_ERROR_OUT_OF_MEMORY, "out of host memory");
+    b->device = d;
+    try {
+        b->pixel_prefix.assign((size_t)count + 1, 0);
+        for (int i = 0; i < count; ++i) {
+            cih_reader r;
+            if (cih_reader_init(&r, streams[i], bytes[i]) != 0) {
+                destroy_batch(b);
+                return fail(CIH_ERROR_MALFORMED, "malformed stream header in batch");
+            }
+            cih_canvas *v = new cih_canvas;
+            v->device = d;
+            v->width = r.width;
+            v->height = r.height;
+            b->views.push_back(v);
+            b->pixel_prefix[(size_t)i + 1] =
+                b->pixel_prefix[(size_t)i] + (uint64_t)r.width * (uint64_t)r.height;
+        }
+        uint64_t total = b->pixel_prefix.back();
+        if ((s = b->bitmaps.reserve(16 * total)) || (s = b->srgb.reserve(4 * total)) ||
+            (s = b->prefix_dev.reserve(8 * b->pixel_prefix.size())) ||
+            (s = b->all_canvases.reserve(sizeof(canvas_desc) * (size_t)count))) {
+            destroy_batch(b);
+            return s;
+        }
+        std::vector<canvas_desc> table((size_t)count);
+        for (int i = 0; i < count; ++i) {
+            cih_canvas *v = b->views[(size_t)i];
+            v->pixels = (char *)b->bitmaps.ptr + 16 * b->pixel_prefix[(size_t)i];
+            table[(size_t)i].bitmap = (uint64_t)(uintptr_t)v->pixels;
+            table[(size_t)i].width = v->width;
+            table[(size_t)i].height = v->height;
+            table[(size_t)i].first_draw = 0;
+            table[(size_t)i].ndraws = 0;
+        }
+        // host state machine: replay every stream into packets, in parallel
+        unsigned workers = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        workers = std::min<unsigned>(workers, (unsigned)count);
+        std::vector<int> status(workers, CIH_OK);
+        std::vector<std::thread> pool;
+        size_t per = ((size_t)count + workers - 1) / workers;
+        for (unsigned w = 0; w < workers; ++w) {
+            size_t first = w * per, last = std::min((size_t)count, first + per);
+            pool.emplace_back(replay_range, b, streams, bytes, first, last, &status[w]);
+        }
+        for (std::thread &t : pool)
+            t.join();
+        for (int st : status)
+            if (st != CIH_OK) {
+                destroy_batch(b);
+                return fail(st, "a batch stream failed to replay");
+            }
+        // decode all packets into one resident draw list
+        std::vector<unsigned char> payload;
+        std::vector<size_t> first_draw((size_t)count + 1, 0);
+        for (int i = 0; i < count; ++i) {
+            cih_canvas *v = b->views[(size_t)i];
+            size_t base = payload.size();
+            payload.insert(payload.end(), v->pending.begin(), v->pending.end());
+            size_t at = base;
+            while (at < payload.size()) {
+                draw_desc desc;
+                size_t used = decode_packet(payload.data(), at, payload.size(), desc, d);
+                if (!used) {
+                    destroy_batch(b);
+                    return fail(CIH_ERROR_MALFORMED, "malformed draw packet in batch");
+                }
+                desc.canvas = i;
+                b->descs.push_back(desc);
+                at += used;
+            }
+            std::vector<unsigned char>().swap(v->pending);
+            first_draw[(size_t)i + 1] = b->descs.size();
+        }
+        // frames: consecutive draw ranges of bounded size; canvas lists per frame
+        const size_t max_draws = 32768;
+        const uint64_t shadow_budget = (uint64_t)1 << 30;
+        std::vector<canvas_desc> frame_canvases;
+        std::vector<uint32_t> frame_items;
+        size_t draw = 0;
+        while (draw < b->descs.size()) {
+            size_t last = draw;
+            uint64_t shadow = 0;
+            while (last < b->descs.size() && last - draw < max_draws) {
+                const draw_desc &dd = b->descs[last];
+                uint64_t need = 0;
+                if (dd.shadow) {
+                    const cih_canvas *v = b->views[(size_t)dd.canvas];
+                    int bo = dd.border;
+                    need = 2ull * (uint64_t)(v->width + 2 * bo + 1) * (uint64_t)(v->height + 2 * bo + 1);
+                }
+                if (last > draw && shadow + need > shadow_budget)
+                    break;
+                shadow += need;
+                ++last;
+            }
+            cih_batch::frame_rec f;
+            f.first_draw = draw;
+            f.ndraws = last - draw;
+            f.canvases_at = frame_canvases.size();
+            int c0 = b->descs[draw].canvas, c1 = b->descs[last - 1].canvas;
+            for (int c = c0; c <= c1; ++c) {
+                size_t lo = std::max(first_draw[(size_t)c], draw);
+                size_t hi = std::min(first_draw[(size_t)c + 1], last);
+                if (hi <= lo)
+                    continue;
+                canvas_desc cd = table[(size_t)c];
+                cd.first_draw = (int32_t)(lo - draw);
+                cd.ndraws = (int32_t)(hi - lo);
+                for (size_t k = lo; k < hi; ++k) // draw_desc.canvas: index in the frame
+                    b->descs[k].canvas = (int32_t)(frame_canvases.size() - f.canvases_at);
+                frame_canvases.push_back(cd);
+            }
+            f.ncanvases = frame_canvases.size() - f.canvases_at;
+            f.shape = shape_of(b->descs.data() + draw, frame_canvases.data() + f.canvases_at,
+                               f.ncanvases);
+            f.items_at = frame_items.size();
+            frame_items.insert(frame_items.end(), f.shape.items.begin(), f.shape.items.end());
+            b->frames.push_back(f);
+            draw = last;
+        }
+        if ((s = b->payload.reserve(std::max<size_t>(payload.size(), 16))) ||
+            (s = b->draws.reserve(sizeof(draw_desc) * std::max<size_t>(b->descs.size(), 1))) ||
+            (s = b->frame_canvases.reserve(sizeof(canvas_desc) *
+                                           std::max<size_t>(frame_canvases.size(), 1))) ||
+            (s = b->frame_items.reserve(4 * std::max<size_t>(frame_items.size(), 1)))) {
+            destroy_batch(b);
+            return s;
+        }
+        hipError_t e;
+        if ((e = hipMemcpy(b->payload.ptr, payload.data(), payload.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(b->draws.ptr, b->descs.data(), sizeof(draw_desc) * b->descs.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(b->frame_canvases.ptr, frame_canvases.data(),
+                           sizeof(canvas_desc) * frame_canvases.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(b->frame_items.ptr, frame_items.data(), 4 * frame_items.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(b->all_canvases.ptr, table.data(), sizeof(canvas_desc) * table.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess ||
+            (e = hipMemcpy(b->prefix_dev.ptr, b->pixel_prefix.data(),
+                           8 * b->pixel_prefix.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+            destroy_batch(b);
+            return hip_fail(e, "batch upload");
+        }
+    } catch (const std::bad_alloc &) {
+        destroy_batch(b);
+        return fail(CIH_ERROR_OUT_OF_MEMORY, "out of host memory");
+    } catch (const std::system_error &) {
+        destroy_batch(b);
+        return fail(CIH_ERROR_DEVICE, "could not start replay threads");
+    }
+    *out = b;
+    return CIH_OK;
+}
+
+int cih_batch_execute(cih_batch *b, int to_host) {
+    if (!b)
+        return fail(CIH_ERROR_INVALID, "batch is NULL");
+    cih_device *d = b->device;
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    int s;
+    if ((s = set_device(d)))
+        return s;
+    if (!b->ran) {
+        // first run: checked frames size the arenas; then capture the graph
+        if ((s = enqueue_batch(b, true)))
+            return s;
+        b->ran = true;
+        if (b->use_graph && !d->profiling) {
+            HIP_TRY(hipStreamSynchronize(d->stream));
+            d->capturing = true;
+            hipError_t e = hipStreamBeginCapture(d->stream, hipStreamCaptureModeRelaxed);
+            if (e == hipSuccess) {
+                s = enqueue_batch(b, false);
+                e = hipStreamEndCapture(d->stream, &b->graph);
+            }
+            d->capturing = false;
+            if (s)
+                return s;
+            if (e == hipSuccess)
+                e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                b->use_graph = false; // fall back to stream launches of the same kernels
+                b->exec = nullptr;
+            }
+        }
+    } else if (b->exec && !d->profiling) {
+        HIP_TRY(hipGraphLaunch(b->exec, d->stream));
+    } else if ((s = enqueue_batch(b, false))) {
+        return s;
+    }
+    if (to_host) {
+        uint64_t total = b->pixel_prefix.back();
+        if ((s = b->srgb_host.reserve(4 * total)))
+            return s;
+        HIP_TRY(hipMemcpyAsync(b->srgb_host.ptr, b->srgb.ptr, 4 * total, hipMemcpyDeviceToHost,
+                               d->stream));
+        HIP_TRY(hipStreamSynchronize(d->stream));
+    }
+    return CIH_OK;
+}
+
+int cih_batch_size(cih_batch *b, int index, int *width, int *height) {
+    if (!b || index < 0 || (size_t)index >= b->views.size())
+        return fail(CIH_ERROR_INVALID, "invalid batch index");
+    if (width)
+        *width = b->views[(size_t)index]->width;
+    if (height)
+        *height = b->views[(size_t)index]->height;
+    return CIH_OK;
+}
+
+int cih_batch_image(cih_batch *b, int index, unsigned char *image) {
+    if (!b || !image || index < 0 || (size_t)index >= b->views.size())
+        return fail(CIH_ERROR_INVALID, "invalid batch image arguments");
+    if (!b->srgb_host.ptr)
+        return fail(CIH_ERROR_INVALID, "execute the batch with to_host first");
+    std::memcpy(image, (const unsigned char *)b->srgb_host.ptr + 4 * b->pixel_prefix[(size_t)index],
+                4 * (size_t)(b->pixel_prefix[(size_t)index + 1] - b->pixel_prefix[(size_t)index]));
+    return CIH_OK;
+}
+
+int cih_batch_linear(cih_batch *b, int index, float *rgba) {
+    if (!b || !rgba || index < 0 || (size_t)index >= b->views.size())
+        return fail(CIH_ERROR_INVALID, "invalid batch arguments");
+    cih_device *d = b->device;
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    const cih_canvas *v = b->views[(size_t)index];
+    HIP_TRY(hipMemcpyAsync(rgba, v->pixels, 16 * (size_t)v->width * v->height,
+                           hipMemcpyDeviceToHost, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    return CIH_OK;
+}
+
+int cih_batch_stats(cih_batch *b, uint64_t *stats) {
+    if (!b || !stats)
+        return fail(CIH_ERROR_INVALID, "NULL argument");
+    stats[0] = b->views.size();
+    stats[1] = b->descs.size();
+    stats[2] = b->frames.size();
+    stats[3] = b->pixel_prefix.back();
+    return CIH_OK;
+}
+
+int cih_batch_destroy(cih_batch *b) {
+    if (!b)
+        return fail(CIH_ERROR_INVALID, "batch is NULL");
+    cih_device *d = b->device;
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    set_device(d);
+    hipStreamSynchronize(d->stream);
+    destroy_batch(b);
     return CIH_OK;
 }
 

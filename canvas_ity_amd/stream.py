@@ -88,3 +88,42 @@ class Stream:
 
     def tobytes(self):
         return np.asarray(self.words, dtype=np.uint32).tobytes()
+
+
+ARITY = {1: 2, 2: 1, 3: 2, 4: 6, 5: 6, 6: 1, 7: 1, 8: 4, 9: 1, 10: 2, 11: 1, 12: 1, 13: 1, 14: 1,
+         16: 1, 17: 5, 18: 5, 19: 7, 20: 6, 21: 3, 22: 0, 23: 2, 24: 0, 25: 2, 26: 4, 27: 6,
+         28: 5, 29: 6, 30: 4, 31: 0, 32: 0, 33: 0, 34: 4, 35: 4, 36: 4, 37: 5, 38: 0, 39: 0,
+         41: 3, 42: 1}
+DRAW_OPS = (31, 32, 34, 35, 36, 37)  # fill, stroke, clear/fill/stroke_rectangle, draw_image
+
+
+def records(stream):
+    """[(opcode, word offset of the opcode, argument count)] of a stream."""
+    words = np.frombuffer(stream, dtype=np.uint32)
+    at, out = 4, []
+    while at < len(words):
+        code = int(words[at])
+        if code == 15:
+            n = 1 + max(int(np.int32(words[at + 1])), 0)
+        elif code == 40:
+            n = 3 + int(words[at + 2]) * int(words[at + 3])
+        else:
+            n = ARITY[code]
+        out.append((code, at, n))
+        at += 1 + n
+    return out
+
+
+def count_draws(stream):
+    return sum(1 for code, _, _ in records(stream) if code in DRAW_OPS)
+
+
+def truncate_draws(stream, draws):
+    """The stream up to and including its `draws`-th drawing call."""
+    seen = 0
+    for code, at, n in records(stream):
+        if code in DRAW_OPS:
+            seen += 1
+            if seen == draws:
+                return bytes(stream[:4 * (at + 1 + n)])
+    return bytes(stream)

@@ -143,6 +143,25 @@ CIH_API int cih_canvas_put_image_data(cih_canvas *canvas, const unsigned char *i
  * (the reference's private `bitmap`, canvas_ity.hpp:1171) to rgba. */
 CIH_API int cih_canvas_read_linear(cih_canvas *canvas, float *rgba);
 
+/* Runs the canvas' last flushed draw list again from HBM (optionally after
+ * clearing the canvas, optionally followed by the sRGB8 conversion into a
+ * device buffer), asynchronously.  For benchmarks: the inputs stay resident
+ * and nothing crosses PCIe.  No reference counterpart. */
+CIH_API int cih_canvas_execute(cih_canvas *canvas, int clear, int readback);
+
+/* Work counts of the canvas' last draw list, from one synchronous run of it
+ * (the canvas is cleared first): stats[0] pixels composited (draw and shadow
+ * layers), stats[1] coverage runs, stats[2] polygon/scratch points,
+ * stats[3] shadow image floats.  Benchmarks use them for algorithmic bytes. */
+CIH_API int cih_canvas_stats(cih_canvas *canvas, uint64_t *stats);
+
+/* Kernel timing: while enabled, every frame records HIP events around each
+ * of its kernels on the device's stream; cih_device_profile_read waits for the
+ * device and writes {"frames": n, "<kernel>": total_ms, ...} as JSON, then
+ * restarts the count.  No reference counterpart. */
+CIH_API int cih_device_profile(cih_device *device, int enable);
+CIH_API int cih_device_profile_read(cih_device *device, char *json, size_t capacity);
+
 /* Test seams for stage-level differential tests.  They read the last frame
  * run on the canvas' device, so call them right after a flush whose draws fit
  * one frame and before any other canvas on the device is flushed.
@@ -162,6 +181,29 @@ CIH_API int cih_trace_runs(cih_canvas *canvas, int draw, int pass, int32_t *xs, 
 /* Replays a command stream (cih_stream.h) through canvas_ity_hip.hpp's
  * canvas API into this canvas.  The stream header's size must match. */
 CIH_API int cih_canvas_replay(cih_canvas *canvas, const void *stream, size_t bytes);
+
+/* ---- batches: many independent canvases in shared frames ------------------
+ * A batch replays every command stream through canvas_ity_hip.hpp (host
+ * threads in parallel), keeps all canvases, draw records and packets resident
+ * in HBM, and renders them with frames that each hold many canvases' draws.
+ * The first execute sizes the arenas and captures the launch sequence in a
+ * HIP graph; later executes replay the graph.  Every execute starts from
+ * cleared canvases (as fresh canvas objects would) and ends with the sRGB8
+ * conversion of every canvas into one packed device buffer; `to_host` copies
+ * it to pinned host memory and waits.  No reference counterpart: the
+ * reference renders one canvas per object (canvas_ity.hpp:194). */
+typedef struct cih_batch cih_batch;
+CIH_API int cih_batch_create(cih_device *device, int count, const void *const *streams,
+                             const size_t *bytes, cih_batch **batch);
+CIH_API int cih_batch_execute(cih_batch *batch, int to_host);
+CIH_API int cih_batch_size(cih_batch *batch, int index, int *width, int *height);
+/* Copies canvas `index`'s width*height*4 sRGB8 bytes of the last execute. */
+CIH_API int cih_batch_image(cih_batch *batch, int index, unsigned char *image);
+/* Copies canvas `index`'s fp32 linear premultiplied bitmap (test seam). */
+CIH_API int cih_batch_linear(cih_batch *batch, int index, float *rgba);
+/* stats: canvases, draws, frames, pixels. */
+CIH_API int cih_batch_stats(cih_batch *batch, uint64_t *stats);
+CIH_API int cih_batch_destroy(cih_batch *batch);
 
 /* ---- synthetic workloads ----------------------------------------------------
  * Generates the seeded command stream of a benchmark configuration:

@@ -1,0 +1,98 @@
+"""The C-ABI library builds, loads and exports every declared entry point;
+the command stream format and the seeded generator behave (no GPU needed)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import canvas_ity_amd as cia
+from canvas_ity_amd.stream import Stream
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"CIH_API\s+[\w\s\*]*?\b(cih_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = cia.library()
+    names = declared("cih_abi.h")
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in cia.SIGNATURES, name
+
+
+def test_no_device_is_reported_cleanly():
+    # In a container without a GPU, opening a device fails with a status code
+    # and a message instead of crashing; on a GPU machine it succeeds.
+    import ctypes
+    handle = ctypes.c_void_p()
+    status = cia.library().cih_device_open(0, ctypes.byref(handle))
+    if status == 0:
+        cia.library().cih_device_close(handle)
+    else:
+        assert status == -3
+        assert b"device" in cia.library().cih_last_error()
+
+
+def test_scene_generator_is_deterministic():
+    a = cia.scene(1, 0xC1)
+    b = cia.scene(1, 0xC1)
+    c = cia.scene(1, 0xC2)
+    assert a == b and a != c
+    assert cia.stream_size(a) == (512, 512)
+    assert cia.stream_size(cia.scene(2, 1)) == (2048, 2048)
+    assert cia.stream_size(cia.scene(3, 1)) == (1024, 1024)
+
+
+@pytest.mark.parametrize("index", range(8))
+def test_batch_and_serve_sizes(index):
+    w, h = cia.stream_size(cia.scene(4, cia.scene_seed(0xC4, index)))
+    assert 64 <= w <= 1024 and 64 <= h <= 1024
+    w, h = cia.stream_size(cia.scene(5, cia.scene_seed(0xC5, index)))
+    assert 128 <= w <= 2048 and 128 <= h <= 2048
+
+
+def count_ops(stream, op):
+    from canvas_ity_amd.stream import OPS
+    words = np.frombuffer(stream, dtype=np.uint32)
+    arity = {1: 2, 2: 1, 3: 2, 4: 6, 5: 6, 6: 1, 7: 1, 8: 4, 9: 1, 10: 2, 11: 1, 12: 1, 13: 1,
+             14: 1, 16: 1, 17: 5, 18: 5, 19: 7, 20: 6, 21: 3, 22: 0, 23: 2, 24: 0, 25: 2,
+             26: 4, 27: 6, 28: 5, 29: 6, 30: 4, 31: 0, 32: 0, 33: 0, 34: 4, 35: 4, 36: 4,
+             37: 5, 38: 0, 39: 0, 41: 3, 42: 1}
+    at, n = 4, 0
+    while at < len(words):
+        code = int(words[at])
+        if code == 15:
+            size = 1 + max(int(np.int32(words[at + 1])), 0)
+        elif code == 40:
+            size = 3 + int(words[at + 2]) * int(words[at + 3])
+        else:
+            size = arity[code]
+        n += code == OPS[op]
+        at += 1 + size
+    assert at == len(words)
+    return n
+
+
+def test_configuration_draw_counts():
+    s1 = cia.scene(1, 0xC1)
+    assert count_ops(s1, "fill") == 200 and count_ops(s1, "stroke") == 50
+    s2 = cia.scene(2, 0xC2)
+    assert count_ops(s2, "fill") + count_ops(s2, "stroke") == 5000
+    s3 = cia.scene(3, 0xC3)
+    assert count_ops(s3, "draw_image") == 100
+    assert count_ops(s3, "fill") + count_ops(s3, "stroke") == 500
+
+
+def test_stream_writer_round_trip():
+    s = Stream(16, 8).set_color(0, 1, 0.5, 0.25, 1).fill_rectangle(1, 2, 3, 4)
+    s.set_line_dash([1.5, 2.5]).set_line_dash(None)
+    words = np.frombuffer(s.tobytes(), dtype=np.uint32)
+    assert words[0] == 0x53484943 and words[2] == 16 and words[3] == 8
+    assert words[4] == 17 and np.frombuffer(words[6:10].tobytes(), np.float32)[1] == 0.5
+    assert count_ops(s.tobytes(), "fill_rectangle") == 1

@@ -1,0 +1,159 @@
+"""Pins the CPU oracle (oracle/cih_oracle.c) before it is trusted as checker.
+
+* the reference's exact known-answer test for get_image_data
+  (test.cpp:1800-1830, hash 0xf53f9792);
+* bit-identical fp32 bitmaps and sRGB8 against the reference header compiled
+  from source (oracle/_ref), on seeded scenes of all five configurations and
+  on feature scenes (skipped only where that build is absent);
+* closed forms (tests/scenes.py) that hold independently of canvas_ity's
+  code: rectangle overlap areas, polygon areas, supersampled coverage, the
+  W3C Porter-Duff table, the IEC sRGB transfer with the ordered dither,
+  gradient interpolation, blur mass conservation, identity resampling and
+  stroke areas.  A planted mistake in any of those stages fails one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import canvas_ity_amd as cia
+import oracle
+import scenes
+from canvas_ity_amd.stream import COMPOSITE
+
+REFERENCE = oracle.available("reference")
+
+
+def render(stream, kind="port"):
+    return oracle.render(stream, kind)
+
+
+def test_kat_get_image_data():
+    c = oracle.Canvas(256, 256, "port")
+    c.replay(scenes.kat_stream())
+    h = scenes.kat_hash(lambda out, w, hh, stride, x, y: c.get_image_data(w, hh, stride, x, y,
+                                                                          out=out))
+    assert h == scenes.KAT_HASH
+
+
+@pytest.mark.skipif(not REFERENCE, reason="reference build (oracle/_ref) not present")
+def test_kat_reference_build():
+    c = oracle.Canvas(256, 256, "reference")
+    c.replay(scenes.kat_stream())
+    h = scenes.kat_hash(lambda out, w, hh, stride, x, y: c.get_image_data(w, hh, stride, x, y,
+                                                                          out=out))
+    assert h == scenes.KAT_HASH
+
+
+CONFIG_SCENES = [(1, 0xC1, 0, 0, 0, 0), (2, 0xC2, 384, 320, 250, 0), (2, 0x2C2, 256, 256, 120, 1),
+                 (3, 0xC3, 320, 288, 60, 0), (4, 0xC4, 0, 0, 0, 0), (4, 0x1C4, 0, 0, 0, 0),
+                 (5, 0xC5, 256, 200, 80, 0), (5, 0x1C5, 200, 256, 60, 4)]
+
+
+@pytest.mark.skipif(not REFERENCE, reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("cfg,seed,w,h,n,flags", CONFIG_SCENES)
+def test_oracle_bit_identical_to_reference(cfg, seed, w, h, n, flags):
+    stream = cia.scene(cfg, seed, w, h, n, flags)
+    ref8, refl = render(stream, "reference")
+    got8, gotl = render(stream, "port")
+    if flags & cia.SCENE_EVENODD:
+        pytest.skip("even-odd is an extension the reference does not have")
+    assert np.array_equal(refl.view(np.uint32), gotl.view(np.uint32))
+    assert np.array_equal(ref8, got8)
+
+
+def test_fractional_rectangle_coverage():
+    rect = (3.25, 5.6, 17.3, 9.15)
+    _, lin = render(scenes.fractional_rect(32, rect))
+    exp = scenes.expected_rect_alpha(32, rect)
+    assert np.abs(lin[..., 3] - exp).max() < 2e-6
+    assert np.abs(lin[..., 0] - exp).max() < 2e-6  # white: every channel = coverage
+
+
+def test_polygon_area():
+    _, lin = render(scenes.polygon())
+    assert abs(lin[..., 3].astype(np.float64).sum() - scenes.shoelace(scenes.STAR)) < 5e-4
+
+
+def test_supersampled_triangle():
+    _, lin = render(scenes.triangle(16))
+    exp = scenes.supersampled_coverage(scenes.TRIANGLE, 16, n=64)
+    assert np.abs(lin[..., 3] - exp).max() < 0.03
+    assert abs(lin[..., 3].astype(np.float64).sum() - scenes.shoelace(scenes.TRIANGLE)) < 5e-4
+
+
+def test_orientation_and_integer_translation():
+    pts = scenes.STAR
+    _, a = render(scenes.polygon(pts))
+    _, b = render(scenes.polygon(pts[::-1]))
+    assert np.abs(a - b).max() < 1e-6
+    moved = [(x + 3, y + 2) for x, y in pts]
+    _, c = render(scenes.polygon(moved, 47))
+    assert np.abs(c[2:46, 3:47] - a).max() < 1e-5
+
+
+@pytest.mark.parametrize("op", sorted(scenes.PORTER_DUFF))
+def test_porter_duff_table(op):
+    _, lin = render(scenes.composite_scene(op))
+    assert np.abs(lin - scenes.expected_composite(op)).max() < 2e-6
+
+
+def test_srgb_readback_and_dither():
+    rgba8, _ = render(scenes.solid())
+    exp = scenes.expected_solid_srgb8()
+    # truncation of 255*v + dither; float32 vs float64 may differ by one LSB
+    # only where the value is within rounding of an integer
+    near = np.abs(exp - np.round(exp)) < 1e-3
+    got = rgba8.astype(np.float64)
+    assert np.all((got == np.floor(exp)) | near)
+
+
+def test_linear_gradient():
+    _, lin = render(scenes.linear_gradient())
+    assert np.abs(lin - scenes.expected_linear_gradient()).max() < 2e-6
+
+
+@pytest.mark.parametrize("blur", [0.0, 3.0, 11.5, 32.0])
+def test_shadow_mass_and_symmetry(blur):
+    _, lin = render(scenes.shadow_scene(blur))
+    # the shape covers [6, 20) x [6, 16); its shadow, offset by (80, 76) and
+    # spread by at most 3 (r + 1) <= 48 px, does not overlap it
+    alpha = lin[..., 3].astype(np.float64)
+    shape = scenes.SHADOW_RECT[2] * scenes.SHADOW_RECT[3]
+    assert abs(alpha[:21, :21].sum() - shape) < 1e-4
+    shadow = alpha.copy()
+    shadow[:21, :21] = 0.0
+    assert abs(shadow.sum() - scenes.shadow_mass_expected()) < 2e-3
+    ys, xs = np.nonzero(shadow > 1e-4)
+    assert abs((xs.min() + xs.max()) / 2 + 0.5 - (13 + scenes.SHADOW_OFFSET[0])) <= 0.5 + 1e-9
+    assert abs((ys.min() + ys.max()) / 2 + 0.5 - (11 + scenes.SHADOW_OFFSET[1])) <= 0.5 + 1e-9
+
+
+def test_identity_resampling():
+    img = scenes.noise_image()
+    _, lin = render(scenes.identity_image(img))
+    assert np.abs(lin - scenes.expected_image_linear(img)).max() < 2e-6
+
+
+@pytest.mark.parametrize("cap", [0, 1, 2])
+def test_stroke_area(cap):
+    _, lin = render(scenes.straight_stroke(cap))
+    area = lin[..., 3].astype(np.float64).sum()
+    tol = 1e-3 if cap < 2 else 0.02 * scenes.stroke_area(cap)
+    assert abs(area - scenes.stroke_area(cap)) < tol
+
+
+def test_even_odd_extension():
+    _, nz = render(scenes.nested_squares(False))
+    _, eo = render(scenes.nested_squares(True))
+    assert np.all(nz[12:20, 12:20, 3] == 1.0)
+    assert np.all(eo[12:20, 12:20, 3] == 0.0)
+    assert np.all(eo[5:9, 5:27, 3] == 1.0)
+
+
+def test_malformed_stream_rejected():
+    stream = bytearray(scenes.solid())
+    stream[16:20] = (99).to_bytes(4, "little")
+    c = oracle.Canvas(8, 8, "port")
+    with pytest.raises(ValueError):
+        c.replay(bytes(stream))
