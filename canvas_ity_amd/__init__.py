@@ -16,7 +16,7 @@ LIBRARY = os.path.join(HERE, "libcanvas_ity_hip.so")
 
 OK = 0
 ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "device error",
-          -4: "malformed input", -5: "over memory budget"}
+          -4: "malformed input", -5: "over memory budget", -6: "timeout"}
 
 # composite_operation values (canvas_ity.hpp:146-149)
 OPS = {"source_in": 1, "source_copy": 2, "source_out": 3, "destination_in": 4,
@@ -61,6 +61,13 @@ SIGNATURES = {
     "cih_batch_linear": (_c.c_int, [_P, _c.c_int, _P]),
     "cih_batch_stats": (_c.c_int, [_P, _c.POINTER(_c.c_uint64)]),
     "cih_batch_destroy": (_c.c_int, [_P]),
+    "cih_server_start": (_c.c_int, [_P, _c.c_uint64, _c.c_int, _c.c_int, _c.POINTER(_P)]),
+    "cih_server_submit": (_c.c_int, [_P, _P, _c.c_size_t, _c.POINTER(_c.c_uint64)]),
+    "cih_server_wait": (_c.c_int, [_P, _c.POINTER(_c.c_uint64), _c.c_int]),
+    "cih_server_result": (_c.c_int, [_P, _c.c_uint64, _P, _c.POINTER(_c.c_int),
+                                     _c.POINTER(_c.c_int), _c.POINTER(_c.c_double)]),
+    "cih_server_stats": (_c.c_int, [_P, _c.POINTER(_c.c_uint64)]),
+    "cih_server_stop": (_c.c_int, [_P]),
     "cih_scene_generate": (_c.c_int, [_c.c_int, _c.c_uint64, _c.c_int, _c.c_int, _c.c_int,
                                       _c.c_int, _c.POINTER(_P), _c.POINTER(_c.c_size_t)]),
     "cih_scene_seed": (_c.c_uint64, [_c.c_uint64, _c.c_uint64]),
@@ -299,6 +306,60 @@ class Batch:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop (cih_server_* in include/cih_abi.h)."""
+
+    def __init__(self, hbm_budget=64 << 30, max_batch=64, workers=0, device=None):
+        self.device = device or default_device()
+        self.handle = _P()
+        _check(library().cih_server_start(self.device.handle, hbm_budget, max_batch, workers,
+                                          _c.byref(self.handle)), "cih_server_start")
+        self.sizes = {}
+
+    def submit(self, stream):
+        rid = _c.c_uint64()
+        buf = _c.create_string_buffer(bytes(stream), len(stream))
+        _check(library().cih_server_submit(self.handle, buf, len(stream), _c.byref(rid)),
+               "cih_server_submit")
+        self.sizes[rid.value] = stream_size(stream)
+        return rid.value
+
+    def wait(self, timeout_ms=1000):
+        """Next finished request id, or None on timeout."""
+        rid = _c.c_uint64()
+        status = library().cih_server_wait(self.handle, _c.byref(rid), timeout_ms)
+        if status == -6:
+            return None
+        _check(status, "cih_server_wait")
+        return rid.value
+
+    def result(self, rid):
+        """(sRGB8 image, latency in ms) of a finished request."""
+        w, h = self.sizes.pop(rid)
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        lat = _c.c_double()
+        _check(library().cih_server_result(self.handle, rid, out.ctypes.data_as(_P), None, None,
+                                           _c.byref(lat)), "cih_server_result")
+        return out, lat.value
+
+    def stats(self):
+        out = (_c.c_uint64 * 4)()
+        _check(library().cih_server_stats(self.handle, out), "cih_server_stats")
+        return {"served": out[0], "batches": out[1], "max_in_flight": out[2],
+                "reruns": out[3]}
+
+    def stop(self):
+        if self.handle:
+            library().cih_server_stop(self.handle)
+            self.handle = _P()
+
+    def __del__(self):
+        try:
+            self.stop()
         except Exception:
             pass
 

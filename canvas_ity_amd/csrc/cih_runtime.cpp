@@ -21,6 +21,10 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
 #include <system_error>
 #include <thread>
 #include <string>
@@ -152,15 +156,28 @@ struct image_entry {
 
 } // namespace
 
+// A stream with its own arenas and frame buffers: frames on different lanes
+// can be in flight together.
+struct exec_lane {
+    hipStream_t stream = nullptr;
+    dbuf points, words, runs, shadow, counters; // arenas and their counters
+    dbuf state, canvases, items;                 // frame inputs and per-draw results
+    hbuf host_counters;
+    size_t last_ndraws = 0; // draws of the lane's last frame (for the trace seams)
+    void release() {
+        dbuf *all[] = {&points, &words, &runs, &shadow, &counters, &state, &canvases, &items};
+        for (dbuf *b : all)
+            b->release();
+        host_counters.release();
+    }
+};
+
 struct cih_device {
     int ordinal = 0;
     hipStream_t stream = nullptr;
     std::recursive_mutex mutex;
-    // arenas
-    dbuf points, words, runs, shadow, counters;
-    // frame inputs
-    dbuf state, canvases, items;
-    hbuf staging, host_counters;
+    exec_lane main; // the lane of canvases and batches (stream == `stream`)
+    hbuf staging;
     // images
     std::vector<image_entry *> images;
     std::unordered_multimap<uint64_t, int32_t> image_index;
@@ -170,7 +187,6 @@ struct cih_device {
     dbuf readback;
     hbuf readback_host;
     size_t frames = 0, retries = 0;
-    size_t last_ndraws = 0; // draws of the last frame (for the trace seams)
     // kernel timing with HIP events on the device's stream
     bool profiling = false;
     bool capturing = false; // a HIP graph capture is recording on the stream
@@ -297,10 +313,10 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
     return p.bytes;
 }
 
-int ensure_arenas(cih_device *d, uint64_t pts, uint64_t words, uint64_t runs, uint64_t shadow) {
+int ensure_arenas(exec_lane &L, uint64_t pts, uint64_t words, uint64_t runs, uint64_t shadow) {
     int s;
-    if ((s = d->points.reserve(8 * pts)) || (s = d->words.reserve(4 * words)) ||
-        (s = d->runs.reserve(sizeof(run) * runs)) || (s = d->shadow.reserve(4 * shadow)))
+    if ((s = L.points.reserve(8 * pts)) || (s = L.words.reserve(4 * words)) ||
+        (s = L.runs.reserve(sizeof(run) * runs)) || (s = L.shadow.reserve(4 * shadow)))
         return s;
     return CIH_OK;
 }
@@ -366,43 +382,44 @@ frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t 
 // the arenas to what the frame asked for and runs it again (kernels write no
 // pixel once an arena has overflowed).  Without it nothing synchronises or
 // allocates (arenas must already fit: graph capture relies on that).
-int run_kernels(cih_device *d, const unsigned char *payload, const draw_desc *draws, size_t n,
+int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
+                const draw_desc *draws, size_t n,
                 const canvas_desc *canvases_dev, size_t ncanvases, const uint32_t *items_dev,
                 const frame_shape &shape, bool check) {
     int s;
     if (check) {
-        if ((s = d->state.reserve(sizeof(draw_state) * std::max<size_t>(n, 1))) ||
-            (s = d->counters.reserve(64)) || (s = d->host_counters.reserve(64)))
+        if ((s = L.state.reserve(sizeof(draw_state) * std::max<size_t>(n, 1))) ||
+            (s = L.counters.reserve(64)) || (s = L.host_counters.reserve(64)))
             return s;
         uint64_t want_points = std::max<uint64_t>(1 << 22, 64 * shape.in_points);
         uint64_t want_words = std::max<uint64_t>(1 << 22, 8 * shape.in_points + 64 * n);
         uint64_t want_runs = std::max<uint64_t>(1 << 22, 16 * shape.in_points);
-        if ((s = ensure_arenas(d, std::max<uint64_t>(want_points, d->points.bytes / 8),
-                               std::max<uint64_t>(want_words, d->words.bytes / 4),
-                               std::max<uint64_t>(want_runs, d->runs.bytes / sizeof(run)),
-                               std::max<uint64_t>(shape.shadow_bound, d->shadow.bytes / 4))))
+        if ((s = ensure_arenas(L, std::max<uint64_t>(want_points, L.points.bytes / 8),
+                               std::max<uint64_t>(want_words, L.words.bytes / 4),
+                               std::max<uint64_t>(want_runs, L.runs.bytes / sizeof(run)),
+                               std::max<uint64_t>(shape.shadow_bound, L.shadow.bytes / 4))))
             return s;
     }
-    d->last_ndraws = n;
+    L.last_ndraws = n;
     for (int attempt = 0; attempt < 8; ++attempt) {
         frame F;
         std::memset(&F, 0, sizeof(F));
         F.payload = payload;
         F.draws = draws;
-        F.state = (draw_state *)d->state.ptr;
+        F.state = (draw_state *)L.state.ptr;
         F.canvases = canvases_dev;
         F.images = (const image_desc *)d->image_table.ptr;
         F.ndraws = (int32_t)n;
         F.ncanvases = (int32_t)ncanvases;
-        F.points = (float *)d->points.ptr;
-        F.points_cap = d->points.bytes / 8;
-        F.words = (uint32_t *)d->words.ptr;
-        F.words_cap = d->words.bytes / 4;
-        F.runs = (run *)d->runs.ptr;
-        F.runs_cap = d->runs.bytes / sizeof(run);
-        F.shadow = (float *)d->shadow.ptr;
-        F.shadow_cap = d->shadow.bytes / 4;
-        F.counters = (unsigned long long *)d->counters.ptr;
+        F.points = (float *)L.points.ptr;
+        F.points_cap = L.points.bytes / 8;
+        F.words = (uint32_t *)L.words.ptr;
+        F.words_cap = L.words.bytes / 4;
+        F.runs = (run *)L.runs.ptr;
+        F.runs_cap = L.runs.bytes / sizeof(run);
+        F.shadow = (float *)L.shadow.ptr;
+        F.shadow_cap = L.shadow.bytes / 4;
+        F.counters = (unsigned long long *)L.counters.ptr;
         F.items = items_dev;
         F.nitems = shape.items.back();
         F.segment = fine_segment();
@@ -412,14 +429,14 @@ int run_kernels(cih_device *d, const unsigned char *payload, const draw_desc *dr
                 return s;
         }
         HIP_TRY(launch_frame(F, shape.max_rows, shape.max_cols, shape.any_shadow, events,
-                             d->stream));
+                             L.stream));
         d->frames++;
         if (!check)
             return CIH_OK;
-        HIP_TRY(hipMemcpyAsync(d->host_counters.ptr, d->counters.ptr, 8 * COUNTER_COUNT,
-                               hipMemcpyDeviceToHost, d->stream));
-        HIP_TRY(hipStreamSynchronize(d->stream));
-        const unsigned long long *hc = (const unsigned long long *)d->host_counters.ptr;
+        HIP_TRY(hipMemcpyAsync(L.host_counters.ptr, L.counters.ptr, 8 * COUNTER_COUNT,
+                               hipMemcpyDeviceToHost, L.stream));
+        HIP_TRY(hipStreamSynchronize(L.stream));
+        const unsigned long long *hc = (const unsigned long long *)L.host_counters.ptr;
         if (!hc[COUNTER_OVERFLOW])
             return CIH_OK;
         d->retries++;
@@ -427,7 +444,7 @@ int run_kernels(cih_device *d, const unsigned char *payload, const draw_desc *dr
         uint64_t nw = std::max<uint64_t>(hc[COUNTER_WORDS], F.words_cap);
         uint64_t nr = std::max<uint64_t>(hc[COUNTER_RUNS], F.runs_cap);
         uint64_t ns = std::max<uint64_t>(hc[COUNTER_SHADOW], F.shadow_cap);
-        if ((s = ensure_arenas(d, hc[COUNTER_POINTS] >= F.points_cap ? 2 * np : np,
+        if ((s = ensure_arenas(L, hc[COUNTER_POINTS] >= F.points_cap ? 2 * np : np,
                                hc[COUNTER_WORDS] >= F.words_cap ? 2 * nw : nw,
                                hc[COUNTER_RUNS] >= F.runs_cap ? 2 * nr : nr,
                                hc[COUNTER_SHADOW] >= F.shadow_cap ? 2 * ns : ns)))
@@ -440,16 +457,16 @@ int run_kernels(cih_device *d, const unsigned char *payload, const draw_desc *dr
 int launch(cih_device *d, const frame_input &in, bool check) {
     int s;
     frame_shape shape = shape_of(in.host, in.canvases.data(), in.canvases.size());
-    if ((s = d->canvases.reserve(sizeof(canvas_desc) * in.canvases.size())) ||
-        (s = d->items.reserve(4 * shape.items.size())))
+    if ((s = d->main.canvases.reserve(sizeof(canvas_desc) * in.canvases.size())) ||
+        (s = d->main.items.reserve(4 * shape.items.size())))
         return s;
-    HIP_TRY(hipMemcpyAsync(d->canvases.ptr, in.canvases.data(),
+    HIP_TRY(hipMemcpyAsync(d->main.canvases.ptr, in.canvases.data(),
                            sizeof(canvas_desc) * in.canvases.size(), hipMemcpyHostToDevice,
                            d->stream));
-    HIP_TRY(hipMemcpyAsync(d->items.ptr, shape.items.data(), 4 * shape.items.size(),
+    HIP_TRY(hipMemcpyAsync(d->main.items.ptr, shape.items.data(), 4 * shape.items.size(),
                            hipMemcpyHostToDevice, d->stream));
-    return run_kernels(d, in.payload, in.draws, in.n, (const canvas_desc *)d->canvases.ptr,
-                       in.canvases.size(), (const uint32_t *)d->items.ptr, shape, check);
+    return run_kernels(d, d->main, in.payload, in.draws, in.n, (const canvas_desc *)d->main.canvases.ptr,
+                       in.canvases.size(), (const uint32_t *)d->main.items.ptr, shape, check);
 }
 
 // Decodes pending packets into the canvas' resident program (payload and
@@ -584,6 +601,7 @@ int cih_device_open(int ordinal, cih_device **device) {
         delete d;
         return hip_fail(e, "hipStreamCreate");
     }
+    d->main.stream = d->stream;
     float table[256];
     for (int v = 0; v < 256; ++v)
         table[v] = linearized(v / 255.0f);
@@ -606,14 +624,13 @@ int cih_device_close(cih_device *d) {
         img->texels.release();
         delete img;
     }
-    dbuf *all[] = {&d->points, &d->words, &d->runs, &d->shadow, &d->counters, &d->state,
-                   &d->canvases, &d->items, &d->image_table, &d->lut, &d->upload, &d->readback};
+    d->main.release();
+    dbuf *all[] = {&d->image_table, &d->lut, &d->upload, &d->readback};
     for (hipEvent_t e : d->event_pool)
         hipEventDestroy(e);
     for (dbuf *b : all)
         b->release();
     d->staging.release();
-    d->host_counters.release();
     d->readback_host.release();
     hipStreamDestroy(d->stream);
     delete d;
@@ -896,7 +913,7 @@ int cih_canvas_stats(cih_canvas *c, uint64_t *stats) {
         in.canvases.push_back(cd);
         if ((s = launch(d, in, true)))
             return s;
-        const unsigned long long *hc = (const unsigned long long *)d->host_counters.ptr;
+        const unsigned long long *hc = (const unsigned long long *)d->main.host_counters.ptr;
         stats[0] += hc[COUNTER_PIXEL_DRAWS];
         stats[1] += hc[COUNTER_RUNS];
         stats[2] += hc[COUNTER_POINTS];
@@ -947,10 +964,10 @@ int cih_trace_polygons(cih_canvas *c, int draw, float *points, size_t point_capa
         return fail(CIH_ERROR_INVALID, "invalid trace arguments");
     cih_device *d = c->device;
     std::lock_guard<std::recursive_mutex> lock(d->mutex);
-    if ((size_t)draw >= d->last_ndraws)
+    if ((size_t)draw >= d->main.last_ndraws)
         return fail(CIH_ERROR_INVALID, "no such draw in the last frame");
     draw_state S;
-    HIP_TRY(hipMemcpy(&S, (draw_state *)d->state.ptr + draw, sizeof(S), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&S, (draw_state *)d->main.state.ptr + draw, sizeof(S), hipMemcpyDeviceToHost));
     *npoints = S.poly_npoints;
     *nsubpaths = S.poly_nsub;
     if (points && point_capacity)
@@ -970,23 +987,23 @@ int cih_trace_runs(cih_canvas *c, int draw, int pass, int32_t *xs, int32_t *ys, 
         return fail(CIH_ERROR_INVALID, "invalid trace arguments");
     cih_device *d = c->device;
     std::lock_guard<std::recursive_mutex> lock(d->mutex);
-    if ((size_t)draw >= d->last_ndraws)
+    if ((size_t)draw >= d->main.last_ndraws)
         return fail(CIH_ERROR_INVALID, "no such draw in the last frame");
     draw_state S;
-    HIP_TRY(hipMemcpy(&S, (draw_state *)d->state.ptr + draw, sizeof(S), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&S, (draw_state *)d->main.state.ptr + draw, sizeof(S), hipMemcpyDeviceToHost));
     int row0 = pass ? S.srow0 : S.row0, rows = pass ? S.srows : S.rows;
     uint64_t rowoff = pass ? S.srowoff : S.rowoff, base = pass ? S.sruns : S.runs;
     *nruns = 0;
     if (rows <= 0)
         return CIH_OK;
     std::vector<uint32_t> off((size_t)rows + 1);
-    HIP_TRY(hipMemcpy(off.data(), (uint32_t *)d->words.ptr + rowoff, 4 * off.size(),
+    HIP_TRY(hipMemcpy(off.data(), (uint32_t *)d->main.words.ptr + rowoff, 4 * off.size(),
                       hipMemcpyDeviceToHost));
     size_t total = off.back();
     *nruns = total;
     std::vector<run> r(total);
     if (total)
-        HIP_TRY(hipMemcpy(r.data(), (run *)d->runs.ptr + base, sizeof(run) * total,
+        HIP_TRY(hipMemcpy(r.data(), (run *)d->main.runs.ptr + base, sizeof(run) * total,
                           hipMemcpyDeviceToHost));
     size_t k = 0;
     for (int row = 0; row < rows; ++row)
@@ -1063,7 +1080,7 @@ int enqueue_batch(cih_batch *b, bool check) {
     uint64_t total = b->pixel_prefix.back();
     HIP_TRY(launch_clear_batch(b->all_canvases.ptr, b->prefix_dev.ptr, (int)n, total, d->stream));
     for (const cih_batch::frame_rec &f : b->frames) {
-        if ((s = run_kernels(d, (const unsigned char *)b->payload.ptr,
+        if ((s = run_kernels(d, d->main, (const unsigned char *)b->payload.ptr,
                              (const draw_desc *)b->draws.ptr + f.first_draw, f.ndraws,
                              (const canvas_desc *)b->frame_canvases.ptr + f.canvases_at,
                              f.ncanvases, (const uint32_t *)b->frame_items.ptr + f.items_at,
@@ -1345,6 +1362,580 @@ int cih_batch_destroy(cih_batch *b) {
     set_device(d);
     hipStreamSynchronize(d->stream);
     destroy_batch(b);
+    return CIH_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Serving: continuous batching of requests over two execution lanes.
+//
+// submit() copies a request's command stream and queues it; worker threads
+// replay it through canvas_ity_hip.hpp into draw packets (host state machine)
+// and decode them; the scheduler admits prepared requests in arrival order
+// while their canvases fit the HBM pool, launches them as one batch on a free
+// lane, and keeps admitting into the other lane while the first runs.  A
+// finished batch's sRGB8 results are copied out and its pool space freed, so
+// requests that arrived meanwhile get in as soon as memory and a lane free up.
+
+namespace {
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over [0, size) with coalescing of freed ranges.
+struct range_allocator {
+    std::map<uint64_t, uint64_t> free_ranges; // offset -> size
+    void reset(uint64_t size) {
+        free_ranges.clear();
+        if (size)
+            free_ranges[0] = size;
+    }
+    bool alloc(uint64_t n, uint64_t &at) {
+        for (auto it = free_ranges.begin(); it != free_ranges.end(); ++it)
+            if (it->second >= n) {
+                at = it->first;
+                uint64_t rest = it->second - n;
+                free_ranges.erase(it);
+                if (rest)
+                    free_ranges[at + n] = rest;
+                return true;
+            }
+        return false;
+    }
+    void release(uint64_t at, uint64_t n) {
+        auto next = free_ranges.lower_bound(at);
+        if (next != free_ranges.end() && at + n == next->first) {
+            n += next->second;
+            free_ranges.erase(next);
+        }
+        auto it = free_ranges.lower_bound(at);
+        if (it != free_ranges.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == at) {
+                prev->second += n;
+                return;
+            }
+        }
+        free_ranges[at] = n;
+    }
+};
+
+struct server_request {
+    uint64_t id = 0;
+    std::vector<unsigned char> stream, packets, result;
+    std::vector<draw_desc> descs;
+    int width = 0, height = 0, status = CIH_OK;
+    uint64_t pool_at = 0, bytes = 0;
+    double t_submit = 0, t_done = 0;
+};
+
+struct server_batch {
+    std::vector<server_request *> requests;
+    std::vector<uint64_t> pixel_prefix;
+    std::vector<draw_desc> descs;
+    struct frame_rec {
+        size_t first_draw, ndraws, canvases_at, ncanvases, items_at;
+        frame_shape shape;
+    };
+    std::vector<frame_rec> frames;
+    dbuf payload, draws, frame_canvases, frame_items, all_canvases, prefix, srgb;
+    hbuf staging, srgb_host, counters_host;
+    hipEvent_t done = nullptr;
+    void release() {
+        dbuf *all[] = {&payload, &draws, &frame_canvases, &frame_items, &all_canvases, &prefix,
+                       &srgb};
+        for (dbuf *b : all)
+            b->release();
+        staging.release();
+        srgb_host.release();
+        counters_host.release();
+        if (done)
+            hipEventDestroy(done);
+        done = nullptr;
+    }
+};
+
+} // namespace
+
+struct cih_server {
+    cih_device *device = nullptr;
+    uint64_t budget = 0;
+    int max_batch = 0;
+    dbuf pool;
+    range_allocator allocator;
+    exec_lane lanes[2];
+    server_batch *inflight[2] = {nullptr, nullptr};
+    std::mutex mu;
+    std::condition_variable work, done;
+    std::deque<server_request *> arrived, prepared;
+    std::deque<uint64_t> completed;
+    std::unordered_map<uint64_t, server_request *> requests;
+    uint64_t next_id = 1;
+    bool stopping = false;
+    std::vector<std::thread> workers;
+    std::thread scheduler;
+    uint64_t batches = 0, served = 0, retries = 0, max_inflight_requests = 0;
+    int error = CIH_OK;
+};
+
+namespace {
+
+void server_worker(cih_server *sv) {
+    cih_device *d = sv->device;
+    for (;;) {
+        server_request *r;
+        {
+            std::unique_lock<std::mutex> lock(sv->mu);
+            sv->work.wait(lock, [sv] { return sv->stopping || !sv->arrived.empty(); });
+            if (sv->arrived.empty())
+                return;
+            r = sv->arrived.front();
+            sv->arrived.pop_front();
+        }
+        cih_canvas view;
+        view.device = d;
+        view.width = r->width;
+        view.height = r->height;
+        int st = cih_canvas_replay(&view, r->stream.data(), r->stream.size());
+        if (st == CIH_OK) {
+            std::lock_guard<std::recursive_mutex> dlock(d->mutex); // images table
+            r->packets.swap(view.pending);
+            size_t at = 0;
+            while (at < r->packets.size()) {
+                draw_desc desc;
+                size_t used = decode_packet(r->packets.data(), at, r->packets.size(), desc, d);
+                if (!used) {
+                    st = CIH_ERROR_MALFORMED;
+                    break;
+                }
+                r->descs.push_back(desc);
+                at += used;
+            }
+        }
+        std::vector<unsigned char>().swap(r->stream);
+        std::lock_guard<std::mutex> lock(sv->mu);
+        r->status = st;
+        if (st != CIH_OK) {
+            r->t_done = now_ms();
+            sv->completed.push_back(r->id);
+            sv->done.notify_all();
+        } else {
+            sv->prepared.push_back(r);
+            sv->work.notify_all();
+        }
+    }
+}
+
+// Builds and launches a batch on lane k (caller holds sv->mu).
+int launch_server_batch(cih_server *sv, int k, server_batch *b) {
+    cih_device *d = sv->device;
+    exec_lane &L = sv->lanes[k];
+    int s;
+    size_t n = b->requests.size();
+    b->pixel_prefix.assign(n + 1, 0);
+    std::vector<canvas_desc> table(n);
+    std::vector<unsigned char> payload;
+    std::vector<size_t> first_draw(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) {
+        server_request *r = b->requests[i];
+        b->pixel_prefix[i + 1] = b->pixel_prefix[i] + (uint64_t)r->width * r->height;
+        table[i].bitmap = (uint64_t)(uintptr_t)((char *)sv->pool.ptr + r->pool_at);
+        table[i].width = r->width;
+        table[i].height = r->height;
+        table[i].first_draw = 0;
+        table[i].ndraws = 0;
+        uint32_t base = (uint32_t)payload.size();
+        payload.insert(payload.end(), r->packets.begin(), r->packets.end());
+        for (draw_desc dd : r->descs) { // packet offsets move with the concatenation
+            dd.points_at += base, dd.subpaths_at += base, dd.dash_at += base;
+            dd.colors_at += base, dd.stops_at += base;
+            dd.canvas = (int32_t)i;
+            b->descs.push_back(dd);
+        }
+        first_draw[i + 1] = b->descs.size();
+    }
+    std::vector<canvas_desc> frame_canvases;
+    std::vector<uint32_t> frame_items;
+    size_t draw = 0;
+    const size_t max_draws = 32768;
+    while (draw < b->descs.size()) {
+        size_t last = std::min(b->descs.size(), draw + max_draws);
+        server_batch::frame_rec f;
+        f.first_draw = draw;
+        f.ndraws = last - draw;
+        f.canvases_at = frame_canvases.size();
+        int c0 = b->descs[draw].canvas, c1 = b->descs[last - 1].canvas;
+        for (int c = c0; c <= c1; ++c) {
+            size_t lo = std::max(first_draw[(size_t)c], draw);
+            size_t hi = std::min(first_draw[(size_t)c + 1], last);
+            if (hi <= lo)
+                continue;
+            canvas_desc cd = table[(size_t)c];
+            cd.first_draw = (int32_t)(lo - draw);
+            cd.ndraws = (int32_t)(hi - lo);
+            for (size_t q = lo; q < hi; ++q)
+                b->descs[q].canvas = (int32_t)(frame_canvases.size() - f.canvases_at);
+            frame_canvases.push_back(cd);
+        }
+        f.ncanvases = frame_canvases.size() - f.canvases_at;
+        f.shape = shape_of(b->descs.data() + draw, frame_canvases.data() + f.canvases_at,
+                           f.ncanvases);
+        f.items_at = frame_items.size();
+        frame_items.insert(frame_items.end(), f.shape.items.begin(), f.shape.items.end());
+        b->frames.push_back(f);
+        draw = last;
+    }
+    uint64_t total = b->pixel_prefix.back();
+    size_t sz_payload = std::max<size_t>(payload.size(), 16);
+    size_t sz_draws = sizeof(draw_desc) * std::max<size_t>(b->descs.size(), 1);
+    size_t sz_fc = sizeof(canvas_desc) * std::max<size_t>(frame_canvases.size(), 1);
+    size_t sz_fi = 4 * std::max<size_t>(frame_items.size(), 1);
+    size_t sz_tab = sizeof(canvas_desc) * n, sz_pre = 8 * (n + 1);
+    if ((s = b->payload.reserve(sz_payload)) || (s = b->draws.reserve(sz_draws)) ||
+        (s = b->frame_canvases.reserve(sz_fc)) || (s = b->frame_items.reserve(sz_fi)) ||
+        (s = b->all_canvases.reserve(sz_tab)) || (s = b->prefix.reserve(sz_pre)) ||
+        (s = b->srgb.reserve(4 * total)) || (s = b->srgb_host.reserve(4 * total)) ||
+        (s = b->counters_host.reserve(64 * std::max<size_t>(b->frames.size(), 1))) ||
+        (s = b->staging.reserve(sz_payload + sz_draws + sz_fc + sz_fi + sz_tab + sz_pre)))
+        return s;
+    unsigned char *st = (unsigned char *)b->staging.ptr;
+    size_t o = 0;
+    auto put = [&](dbuf &dst, const void *src, size_t bytes) -> hipError_t {
+        if (bytes)
+            std::memcpy(st + o, src, bytes);
+        hipError_t e = bytes ? hipMemcpyAsync(dst.ptr, st + o, bytes, hipMemcpyHostToDevice,
+                                              L.stream)
+                             : hipSuccess;
+        o += bytes;
+        return e;
+    };
+    HIP_TRY(put(b->payload, payload.data(), payload.size()));
+    HIP_TRY(put(b->draws, b->descs.data(), sizeof(draw_desc) * b->descs.size()));
+    HIP_TRY(put(b->frame_canvases, frame_canvases.data(), sizeof(canvas_desc) * frame_canvases.size()));
+    HIP_TRY(put(b->frame_items, frame_items.data(), 4 * frame_items.size()));
+    HIP_TRY(put(b->all_canvases, table.data(), sz_tab));
+    HIP_TRY(put(b->prefix, b->pixel_prefix.data(), sz_pre));
+    // size the lane's arenas for the largest frame (overflow is still checked)
+    uint64_t in_points = 0, shadow = 0;
+    size_t ndraws = 0;
+    for (const server_batch::frame_rec &f : b->frames) {
+        in_points = std::max(in_points, f.shape.in_points);
+        shadow = std::max(shadow, f.shape.shadow_bound);
+        ndraws = std::max(ndraws, f.ndraws);
+    }
+    if ((s = L.state.reserve(sizeof(draw_state) * std::max<size_t>(ndraws, 1))) ||
+        (s = L.counters.reserve(64)) ||
+        (s = ensure_arenas(L, std::max<uint64_t>(64 * in_points, 1 << 22),
+                           std::max<uint64_t>(8 * in_points + 64 * ndraws, 1 << 22),
+                           std::max<uint64_t>(16 * in_points, 1 << 22), shadow)))
+        return s;
+    HIP_TRY(launch_clear_batch(b->all_canvases.ptr, b->prefix.ptr, (int)n, total, L.stream));
+    for (size_t f = 0; f < b->frames.size(); ++f) {
+        const server_batch::frame_rec &fr = b->frames[f];
+        if ((s = run_kernels(d, L, (const unsigned char *)b->payload.ptr,
+                             (const draw_desc *)b->draws.ptr + fr.first_draw, fr.ndraws,
+                             (const canvas_desc *)b->frame_canvases.ptr + fr.canvases_at,
+                             fr.ncanvases, (const uint32_t *)b->frame_items.ptr + fr.items_at,
+                             fr.shape, false)))
+            return s;
+        HIP_TRY(hipMemcpyAsync((char *)b->counters_host.ptr + 64 * f, L.counters.ptr, 64,
+                               hipMemcpyDeviceToHost, L.stream));
+    }
+    HIP_TRY(launch_readback_batch(b->all_canvases.ptr, b->prefix.ptr, (int)n, total, b->srgb.ptr,
+                                  L.stream));
+    HIP_TRY(hipMemcpyAsync(b->srgb_host.ptr, b->srgb.ptr, 4 * total, hipMemcpyDeviceToHost,
+                           L.stream));
+    HIP_TRY(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(b->done, L.stream));
+    return CIH_OK;
+}
+
+// A finished batch: re-run if an arena overflowed, else deliver the results.
+int finish_server_batch(cih_server *sv, int k) {
+    server_batch *b = sv->inflight[k];
+    exec_lane &L = sv->lanes[k];
+    bool overflow = false;
+    for (size_t f = 0; f < b->frames.size(); ++f) {
+        const unsigned long long *c =
+            (const unsigned long long *)((const char *)b->counters_host.ptr + 64 * f);
+        if (c[COUNTER_OVERFLOW]) {
+            overflow = true;
+            int s = ensure_arenas(L, 2 * std::max<uint64_t>(c[COUNTER_POINTS], L.points.bytes / 8),
+                                  2 * std::max<uint64_t>(c[COUNTER_WORDS], L.words.bytes / 4),
+                                  2 * std::max<uint64_t>(c[COUNTER_RUNS], L.runs.bytes / sizeof(run)),
+                                  2 * std::max<uint64_t>(c[COUNTER_SHADOW], L.shadow.bytes / 4));
+            if (s)
+                return s;
+        }
+    }
+    if (overflow) {
+        sv->retries++;
+        std::vector<server_request *> requests = b->requests;
+        b->release();
+        *b = server_batch();
+        b->requests = requests;
+        return launch_server_batch(sv, k, b);
+    }
+    double t = now_ms();
+    for (size_t i = 0; i < b->requests.size(); ++i) {
+        server_request *r = b->requests[i];
+        size_t bytes = 4 * (size_t)(b->pixel_prefix[i + 1] - b->pixel_prefix[i]);
+        r->result.assign((const unsigned char *)b->srgb_host.ptr + 4 * b->pixel_prefix[i],
+                         (const unsigned char *)b->srgb_host.ptr + 4 * b->pixel_prefix[i] + bytes);
+        sv->allocator.release(r->pool_at, r->bytes);
+        r->t_done = t;
+        std::vector<unsigned char>().swap(r->packets);
+        std::vector<draw_desc>().swap(r->descs);
+        sv->completed.push_back(r->id);
+        sv->served++;
+    }
+    b->release();
+    delete b;
+    sv->inflight[k] = nullptr;
+    sv->done.notify_all();
+    return CIH_OK;
+}
+
+void server_scheduler(cih_server *sv) {
+    cih_device *d = sv->device;
+    hipSetDevice(d->ordinal);
+    std::unique_lock<std::mutex> lock(sv->mu);
+    for (;;) {
+        bool busy = false;
+        for (int k = 0; k < 2; ++k) {
+            server_batch *b = sv->inflight[k];
+            if (!b)
+                continue;
+            busy = true;
+            hipError_t q = hipEventQuery(b->done);
+            if (q == hipErrorNotReady)
+                continue;
+            int s = q == hipSuccess ? finish_server_batch(sv, k) : CIH_ERROR_DEVICE;
+            if (s != CIH_OK) {
+                sv->error = s;
+                for (server_request *r : b->requests) {
+                    r->status = s;
+                    r->t_done = now_ms();
+                    sv->completed.push_back(r->id);
+                }
+                b->release();
+                delete b;
+                sv->inflight[k] = nullptr;
+                sv->done.notify_all();
+            }
+        }
+        for (int k = 0; k < 2 && !sv->prepared.empty(); ++k) {
+            if (sv->inflight[k])
+                continue;
+            server_batch *b = new server_batch;
+            size_t draws = 0;
+            while (!sv->prepared.empty() && (int)b->requests.size() < sv->max_batch) {
+                server_request *r = sv->prepared.front();
+                if (!b->requests.empty() && draws + r->descs.size() > (1u << 20))
+                    break;
+                if (!sv->allocator.alloc(r->bytes, r->pool_at))
+                    break; // wait for memory: admission by HBM footprint
+                sv->prepared.pop_front();
+                b->requests.push_back(r);
+                draws += r->descs.size();
+            }
+            if (b->requests.empty()) {
+                delete b;
+                break;
+            }
+            sv->inflight[k] = b;
+            sv->batches++;
+            uint64_t inflight = 0;
+            for (int j = 0; j < 2; ++j)
+                if (sv->inflight[j])
+                    inflight += sv->inflight[j]->requests.size();
+            sv->max_inflight_requests = std::max(sv->max_inflight_requests, inflight);
+            int s = launch_server_batch(sv, k, b);
+            if (s != CIH_OK) {
+                sv->error = s;
+                for (server_request *r : b->requests) {
+                    sv->allocator.release(r->pool_at, r->bytes);
+                    r->status = s;
+                    r->t_done = now_ms();
+                    sv->completed.push_back(r->id);
+                }
+                b->release();
+                delete b;
+                sv->inflight[k] = nullptr;
+                sv->done.notify_all();
+            }
+            busy = true;
+        }
+        if (sv->stopping && sv->arrived.empty() && sv->prepared.empty() && !sv->inflight[0] &&
+            !sv->inflight[1])
+            return;
+        if (busy)
+            sv->work.wait_for(lock, std::chrono::microseconds(50));
+        else
+            sv->work.wait_for(lock, std::chrono::milliseconds(5));
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int cih_server_start(cih_device *d, uint64_t hbm_budget, int max_batch, int workers,
+                     cih_server **out) {
+    if (!d || !out || max_batch <= 0 || hbm_budget < (1u << 20))
+        return fail(CIH_ERROR_INVALID, "invalid server arguments");
+    *out = nullptr;
+    std::lock_guard<std::recursive_mutex> dlock(d->mutex);
+    int s;
+    if ((s = set_device(d)))
+        return s;
+    cih_server *sv = new (std::nothrow) cih_server;
+    if (!sv)
+        return fail(CIH_ERROR_OUT_OF_MEMORY, "out of host memory");
+    sv->device = d;
+    sv->budget = hbm_budget;
+    sv->max_batch = max_batch;
+    if ((s = sv->pool.reserve(hbm_budget))) {
+        delete sv;
+        return s;
+    }
+    sv->allocator.reset(hbm_budget);
+    for (int k = 0; k < 2; ++k) {
+        hipError_t e = hipStreamCreateWithFlags(&sv->lanes[k].stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            sv->pool.release();
+            delete sv;
+            return hip_fail(e, "server stream");
+        }
+    }
+    if (workers <= 0)
+        workers = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    try {
+        for (int w = 0; w < workers; ++w)
+            sv->workers.emplace_back(server_worker, sv);
+        sv->scheduler = std::thread(server_scheduler, sv);
+    } catch (const std::system_error &) {
+        {
+            std::lock_guard<std::mutex> lock(sv->mu);
+            sv->stopping = true;
+        }
+        sv->work.notify_all();
+        for (std::thread &t : sv->workers)
+            t.join();
+        return fail(CIH_ERROR_DEVICE, "could not start server threads");
+    }
+    *out = sv;
+    return CIH_OK;
+}
+
+int cih_server_submit(cih_server *sv, const void *stream, size_t bytes, uint64_t *request) {
+    if (!sv || !stream || !request)
+        return fail(CIH_ERROR_INVALID, "invalid submit arguments");
+    cih_reader reader;
+    if (cih_reader_init(&reader, stream, bytes) != 0)
+        return fail(CIH_ERROR_MALFORMED, "malformed command stream header");
+    uint64_t need = 16ull * (uint64_t)reader.width * (uint64_t)reader.height;
+    if (need > sv->budget)
+        return fail(CIH_ERROR_BUDGET, "request does not fit the server's HBM budget");
+    server_request *r = new (std::nothrow) server_request;
+    if (!r)
+        return fail(CIH_ERROR_OUT_OF_MEMORY, "out of host memory");
+    try {
+        r->stream.assign((const unsigned char *)stream, (const unsigned char *)stream + bytes);
+    } catch (const std::bad_alloc &) {
+        delete r;
+        return fail(CIH_ERROR_OUT_OF_MEMORY, "out of host memory");
+    }
+    r->width = reader.width;
+    r->height = reader.height;
+    r->bytes = (need + 255) & ~(uint64_t)255;
+    r->t_submit = now_ms();
+    std::lock_guard<std::mutex> lock(sv->mu);
+    if (sv->stopping) {
+        delete r;
+        return fail(CIH_ERROR_INVALID, "server is stopping");
+    }
+    r->id = sv->next_id++;
+    sv->requests[r->id] = r;
+    sv->arrived.push_back(r);
+    *request = r->id;
+    sv->work.notify_all();
+    return CIH_OK;
+}
+
+int cih_server_wait(cih_server *sv, uint64_t *request, int timeout_ms) {
+    if (!sv || !request)
+        return fail(CIH_ERROR_INVALID, "invalid wait arguments");
+    std::unique_lock<std::mutex> lock(sv->mu);
+    if (!sv->done.wait_for(lock, std::chrono::milliseconds(timeout_ms < 0 ? 0 : timeout_ms),
+                           [sv] { return !sv->completed.empty(); }))
+        return fail(CIH_ERROR_TIMEOUT, "no request completed in time");
+    *request = sv->completed.front();
+    sv->completed.pop_front();
+    return CIH_OK;
+}
+
+int cih_server_result(cih_server *sv, uint64_t request, unsigned char *image, int *width,
+                      int *height, double *latency_ms) {
+    if (!sv)
+        return fail(CIH_ERROR_INVALID, "server is NULL");
+    server_request *r;
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        auto it = sv->requests.find(request);
+        if (it == sv->requests.end() || (it->second->t_done == 0))
+            return fail(CIH_ERROR_INVALID, "unknown or unfinished request");
+        r = it->second;
+        sv->requests.erase(it);
+    }
+    int status = r->status;
+    if (width)
+        *width = r->width;
+    if (height)
+        *height = r->height;
+    if (latency_ms)
+        *latency_ms = r->t_done - r->t_submit;
+    if (image && status == CIH_OK)
+        std::memcpy(image, r->result.data(), r->result.size());
+    delete r;
+    return status == CIH_OK ? CIH_OK : fail(status, "request failed");
+}
+
+int cih_server_stats(cih_server *sv, uint64_t *stats) {
+    if (!sv || !stats)
+        return fail(CIH_ERROR_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> lock(sv->mu);
+    stats[0] = sv->served;
+    stats[1] = sv->batches;
+    stats[2] = sv->max_inflight_requests;
+    stats[3] = sv->retries;
+    return CIH_OK;
+}
+
+int cih_server_stop(cih_server *sv) {
+    if (!sv)
+        return fail(CIH_ERROR_INVALID, "server is NULL");
+    {
+        std::lock_guard<std::mutex> lock(sv->mu);
+        sv->stopping = true;
+    }
+    sv->work.notify_all();
+    for (std::thread &t : sv->workers)
+        t.join();
+    if (sv->scheduler.joinable())
+        sv->scheduler.join();
+    for (int k = 0; k < 2; ++k) {
+        hipStreamSynchronize(sv->lanes[k].stream);
+        sv->lanes[k].release();
+        hipStreamDestroy(sv->lanes[k].stream);
+    }
+    for (auto &kv : sv->requests)
+        delete kv.second;
+    sv->pool.release();
+    delete sv;
     return CIH_OK;
 }
 

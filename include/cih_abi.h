@@ -34,6 +34,7 @@ extern "C" {
 #define CIH_ERROR_DEVICE (-3)        /* HIP runtime error or no device       */
 #define CIH_ERROR_MALFORMED (-4)     /* malformed command stream or packet   */
 #define CIH_ERROR_BUDGET (-5)        /* request exceeds the memory budget    */
+#define CIH_ERROR_TIMEOUT (-6)       /* nothing completed within the timeout */
 
 typedef struct cih_device cih_device;
 typedef struct cih_canvas cih_canvas;
@@ -204,6 +205,28 @@ CIH_API int cih_batch_linear(cih_batch *batch, int index, float *rgba);
 /* stats: canvases, draws, frames, pixels. */
 CIH_API int cih_batch_stats(cih_batch *batch, uint64_t *stats);
 CIH_API int cih_batch_destroy(cih_batch *batch);
+
+/* ---- serving: continuous batching ----------------------------------------
+ * A server takes command streams as they arrive (submit copies them), replays
+ * them on worker threads, admits prepared requests in arrival order while
+ * their canvases fit a pool of `hbm_budget` bytes of HBM, and runs admitted
+ * requests as batches on two device streams, so requests that arrive while
+ * others run are admitted as soon as memory and a stream free up.
+ * cih_server_wait returns the next finished request (CIH_ERROR_TIMEOUT if
+ * none within timeout_ms); cih_server_result copies its sRGB8 pixels
+ * (width*height*4 bytes), reports submit-to-finish latency and releases it.
+ * stats: requests served, batches, most requests in flight, re-runs.
+ * No reference counterpart. */
+typedef struct cih_server cih_server;
+CIH_API int cih_server_start(cih_device *device, uint64_t hbm_budget, int max_batch,
+                             int workers, cih_server **server);
+CIH_API int cih_server_submit(cih_server *server, const void *stream, size_t bytes,
+                              uint64_t *request);
+CIH_API int cih_server_wait(cih_server *server, uint64_t *request, int timeout_ms);
+CIH_API int cih_server_result(cih_server *server, uint64_t request, unsigned char *image,
+                              int *width, int *height, double *latency_ms);
+CIH_API int cih_server_stats(cih_server *server, uint64_t *stats);
+CIH_API int cih_server_stop(cih_server *server);
 
 /* ---- synthetic workloads ----------------------------------------------------
  * Generates the seeded command stream of a benchmark configuration:
