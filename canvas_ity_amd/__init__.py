@@ -51,6 +51,8 @@ SIGNATURES = {
     "cih_device_profile_read": (_c.c_int, [_P, _c.c_char_p, _c.c_size_t]),
     "cih_trace_polygons": (_c.c_int, [_P, _c.c_int, _P, _c.c_size_t, _P, _c.c_size_t,
                                       _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)]),
+    "cih_trace_stage": (_c.c_int, [_P, _c.c_int, _c.c_int, _P, _c.c_size_t, _P, _c.c_size_t,
+                                   _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)]),
     "cih_trace_runs": (_c.c_int, [_P, _c.c_int, _c.c_int, _P, _P, _P, _c.c_size_t,
                                   _c.POINTER(_c.c_size_t)]),
     "cih_batch_create": (_c.c_int, [_P, _c.c_int, _c.POINTER(_P), _c.POINTER(_c.c_size_t),
@@ -229,6 +231,19 @@ class Canvas:
         _check(lib.cih_trace_polygons(self.handle, draw, pts.ctypes.data_as(_P), n.value,
                                       sub.ctypes.data_as(_P), m.value, _c.byref(n),
                                       _c.byref(m)), "cih_trace_polygons")
+        return pts, sub
+
+    def trace_stage(self, draw, stage):
+        """(points, subpaths) of stage 0 flatten, 1 dash, 2 polygons (last frame)."""
+        lib = library()
+        n, m = _c.c_size_t(), _c.c_size_t()
+        _check(lib.cih_trace_stage(self.handle, draw, stage, None, 0, None, 0, _c.byref(n),
+                                   _c.byref(m)), "cih_trace_stage")
+        pts = np.zeros((n.value, 2), dtype=np.float32)
+        sub = np.zeros((m.value, 2), dtype=np.uint32)
+        _check(lib.cih_trace_stage(self.handle, draw, stage, pts.ctypes.data_as(_P), n.value,
+                                   sub.ctypes.data_as(_P), m.value, _c.byref(n), _c.byref(m)),
+               "cih_trace_stage")
         return pts, sub
 
     def trace_runs(self, draw, shadow=False):

@@ -31,6 +31,9 @@ struct draw_state {
     // final polygons: device addresses of xy points and (count, closed) pairs
     uint64_t poly_points, poly_counts;
     uint32_t poly_npoints, poly_nsub;
+    // intermediate polylines (test seams): flattening and dashing output
+    uint64_t flat_points, flat_counts, dash_points, dash_counts;
+    uint32_t flat_npoints, flat_nsub, dash_npoints, dash_nsub;
     // main scan conversion: rows [row0, row0 + rows); run offsets per row
     int32_t row0, rows;
     uint64_t rowoff; // word-arena index of rows + 1 offsets

@@ -44,6 +44,25 @@ CIH_HD float smin(float a, float b) { return (b < a) ? b : a; } // std::min
 CIH_HD f2 normalized(f2 v) { return mul(1.0f / smax(1.0e-6f, len(v)), v); }
 CIH_HD f2 perp(f2 v) { return mk(-v.y, v.x); }
 CIH_HD f2 lerp(f2 from, f2 to, float ratio) { return add(from, mul(ratio, sub(to, from))); }
+// acosf/tanf of round joins.  On the device they are evaluated in double and
+// rounded once, i.e. correctly rounded, which matches the host's glibc float
+// versions except where those are off by an ulp (well under 1% of inputs);
+// on the host the float functions themselves are called.
+CIH_HD float join_acos(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (float)acos((double)v);
+#else
+    return acosf(v);
+#endif
+}
+CIH_HD float join_tan(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return (float)tan((double)v);
+#else
+    return tanf(v);
+#endif
+}
+
 // Affine a,b,c,d,e,f applied as (a*x + c*y) + e, (b*x + d*y) + f.
 CIH_HD f2 apply(const float *m, f2 v) {
     return mk(m[0] * v.x + m[2] * v.y + m[4], m[1] * v.x + m[3] * v.y + m[5]);
@@ -314,8 +333,8 @@ CIH_HD void half_stroke(Sink &out, const f2 *points, uint32_t beginning, uint32_
                 out.point(apply(forward, add(point, offset)));
             else if (join == 2) {
                 float cosine = dot(in_direction, out_direction);
-                float angle = acosf(smin(smax(cosine, -1.0f), 1.0f));
-                float alpha = 4.0f / 3.0f * tanf(0.25f * angle);
+                float angle = join_acos(smin(smax(cosine, -1.0f), 1.0f));
+                float alpha = 4.0f / 3.0f * join_tan(0.25f * angle);
                 out.point(apply(forward, side_in));
                 add_bezier(out, apply(forward, side_in),
                            apply(forward, add(side_in, mul(alpha * half, in_direction))),

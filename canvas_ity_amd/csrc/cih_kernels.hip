@@ -134,17 +134,40 @@ struct count_points {
     __device__ void merge_last(uint32_t, uint32_t) { --subpaths; }
 };
 
+// Writes a thread's polylines into its own region [points, ...) and
+// [subpaths, ...).  The newest subpath entry is held back until the next one
+// starts (or finish()): dashing may still close it or merge it into the
+// subpath's first dash (hpp:1916-1931), and the region was sized by a count
+// pass that saw the merged result, so writing the entry early could land in
+// the next thread's region.
 struct write_points {
     f2 *pts;
     uint32_t *sub; // (count, closed) pairs
     uint32_t points, subpaths; // absolute indices
+    bool pending = false;
+    uint32_t pending_count = 0;
+    bool pending_closed = false;
     __device__ void point(f2 p) { pts[points++] = p; }
+    __device__ void commit() {
+        if (pending) {
+            sub[2 * (subpaths - 1)] = pending_count;
+            sub[2 * (subpaths - 1) + 1] = pending_closed ? 1u : 0u;
+            pending = false;
+        }
+    }
     __device__ void subpath(uint32_t c, bool closed) {
-        sub[2 * subpaths] = c;
-        sub[2 * subpaths + 1] = closed ? 1u : 0u;
+        commit();
+        pending = true;
+        pending_count = c;
+        pending_closed = closed;
         ++subpaths;
     }
-    __device__ void close_last() { sub[2 * (subpaths - 1) + 1] = 1u; }
+    __device__ void close_last() {
+        if (pending)
+            pending_closed = true;
+        else
+            sub[2 * (subpaths - 1) + 1] = 1u;
+    }
     __device__ void reverse(uint32_t a, uint32_t b) { // [a, b)
         while (a + 1 < b) {
             f2 t = pts[a];
@@ -155,7 +178,8 @@ struct write_points {
     }
     // std::rotate of the last dash to the front of the subpath's first dash.
     __device__ void merge_last(uint32_t merge_point, uint32_t merge_subpath) {
-        uint32_t c = sub[2 * (subpaths - 1)];
+        uint32_t c = pending ? pending_count : sub[2 * (subpaths - 1)];
+        pending = false;
         uint32_t middle = points - c;
         reverse(merge_point, middle);
         reverse(middle, points);
@@ -163,6 +187,7 @@ struct write_points {
         sub[2 * merge_subpath] += c;
         --subpaths;
     }
+    __device__ void finish() { commit(); }
 };
 
 struct write_flat {
@@ -292,6 +317,7 @@ __device__ line_path per_subpath(const frame &F, const draw_desc &D, line_path i
         else
             stroke_subpath(wp, src, count, closed, D.line_width, D.miter_limit, D.line_join,
                            D.line_cap, D.forward, D.inverse);
+        wp.finish();
     }
     __syncthreads();
     out.pts = pts;
@@ -310,10 +336,17 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(frame F) {
     const draw_desc &D = F.draws[d];
     line_path lp;
     bool stroke = D.kind == CIH_DRAW_STROKE_PATH || D.kind == CIH_DRAW_STROKE_LINES;
+    draw_state &S = F.state[d];
     if (D.kind == CIH_DRAW_FILL_PATH || D.kind == CIH_DRAW_STROKE_PATH) {
         lp = flatten(F, D, stroke, lds, slot);
         if (!lp.pts && D.nsubpaths)
             return;
+        if (threadIdx.x == 0) {
+            S.flat_points = (uint64_t)(uintptr_t)lp.pts;
+            S.flat_counts = (uint64_t)(uintptr_t)lp.sub;
+            S.flat_npoints = lp.npts;
+            S.flat_nsub = lp.nsub;
+        }
     } else {
         lp.pts = (const f2 *)(F.payload + D.points_at);
         lp.sub = (const uint32_t *)(F.payload + D.subpaths_at);
@@ -325,13 +358,18 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(frame F) {
             lp = per_subpath(F, D, lp, 0, lds, slot);
             if (!lp.pts)
                 return;
+            if (threadIdx.x == 0) {
+                S.dash_points = (uint64_t)(uintptr_t)lp.pts;
+                S.dash_counts = (uint64_t)(uintptr_t)lp.sub;
+                S.dash_npoints = lp.npts;
+                S.dash_nsub = lp.nsub;
+            }
         }
         lp = per_subpath(F, D, lp, 1, lds, slot);
         if (!lp.pts)
             return;
     }
     if (threadIdx.x == 0) {
-        draw_state &S = F.state[d];
         S.poly_points = (uint64_t)(uintptr_t)lp.pts;
         S.poly_counts = (uint64_t)(uintptr_t)lp.sub;
         S.poly_npoints = lp.npts;
@@ -1151,8 +1189,11 @@ __global__ void k_put_image(const uint32_t *rgba8, int w, int h, float4 *bitmap,
     }
 }
 
+// sRGB encode (hpp:1282-1284).  The power is evaluated in double and rounded
+// once (correctly rounded), as close as a device can get to the host's powf.
 __device__ inline float delinearized(float v) {
-    return v < 0.0031308f ? 12.92f * v : 1.055f * powf(v, 1.0f / 2.4f) - 0.055f;
+    return v < 0.0031308f ? 12.92f * v
+                          : 1.055f * (float)pow((double)v, (double)(1.0f / 2.4f)) - 0.055f;
 }
 
 __device__ inline float clamp01(float v) { return smin(smax(v, 0.0f), 1.0f); }

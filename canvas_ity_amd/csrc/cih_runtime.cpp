@@ -960,6 +960,13 @@ int cih_device_profile_read(cih_device *d, char *json, size_t capacity) {
 int cih_trace_polygons(cih_canvas *c, int draw, float *points, size_t point_capacity,
                        uint32_t *subpaths, size_t subpath_capacity, size_t *npoints,
                        size_t *nsubpaths) {
+    return cih_trace_stage(c, draw, 2, points, point_capacity, subpaths, subpath_capacity,
+                           npoints, nsubpaths);
+}
+
+int cih_trace_stage(cih_canvas *c, int draw, int stage, float *points, size_t point_capacity,
+                    uint32_t *subpaths, size_t subpath_capacity, size_t *npoints,
+                    size_t *nsubpaths) {
     if (!c || draw < 0 || !npoints || !nsubpaths)
         return fail(CIH_ERROR_INVALID, "invalid trace arguments");
     cih_device *d = c->device;
@@ -968,16 +975,18 @@ int cih_trace_polygons(cih_canvas *c, int draw, float *points, size_t point_capa
         return fail(CIH_ERROR_INVALID, "no such draw in the last frame");
     draw_state S;
     HIP_TRY(hipMemcpy(&S, (draw_state *)d->main.state.ptr + draw, sizeof(S), hipMemcpyDeviceToHost));
-    *npoints = S.poly_npoints;
-    *nsubpaths = S.poly_nsub;
-    if (points && point_capacity)
-        HIP_TRY(hipMemcpy(points, (const void *)(uintptr_t)S.poly_points,
-                          8 * std::min<size_t>(point_capacity, S.poly_npoints),
+    uint64_t pts = stage == 0 ? S.flat_points : stage == 1 ? S.dash_points : S.poly_points;
+    uint64_t cnt = stage == 0 ? S.flat_counts : stage == 1 ? S.dash_counts : S.poly_counts;
+    size_t np = stage == 0 ? S.flat_npoints : stage == 1 ? S.dash_npoints : S.poly_npoints;
+    size_t ns = stage == 0 ? S.flat_nsub : stage == 1 ? S.dash_nsub : S.poly_nsub;
+    *npoints = np;
+    *nsubpaths = ns;
+    if (points && point_capacity && np)
+        HIP_TRY(hipMemcpy(points, (const void *)(uintptr_t)pts, 8 * std::min(point_capacity, np),
                           hipMemcpyDeviceToHost));
-    if (subpaths && subpath_capacity)
-        HIP_TRY(hipMemcpy(subpaths, (const void *)(uintptr_t)S.poly_counts,
-                          8 * std::min<size_t>(subpath_capacity, S.poly_nsub),
-                          hipMemcpyDeviceToHost));
+    if (subpaths && subpath_capacity && ns)
+        HIP_TRY(hipMemcpy(subpaths, (const void *)(uintptr_t)cnt,
+                          8 * std::min(subpath_capacity, ns), hipMemcpyDeviceToHost));
     return CIH_OK;
 }
 

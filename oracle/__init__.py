@@ -55,6 +55,9 @@ def _load(kind):
         fn.restype = ctypes.c_long
         fn.argtypes = [_P, ctypes.c_int, _P, ctypes.c_long, _P, ctypes.c_long,
                        ctypes.POINTER(ctypes.c_long)]
+        fn = lib.cih_ref_dash_lines
+        fn.restype = ctypes.c_long
+        fn.argtypes = [_P, _P, ctypes.c_long, _P, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
         fn = lib.cih_ref_lines_to_runs
         fn.restype = ctypes.c_long
         fn.argtypes = [_P, ctypes.c_float, ctypes.c_float, ctypes.c_int, _P, ctypes.c_long]
@@ -112,6 +115,17 @@ class Canvas:
         sub = np.zeros((cap // 2, 2), dtype=np.int32)
         ns = ctypes.c_long()
         n = lib.cih_ref_path_lines(self.handle, int(stroking), pts.ctypes.data_as(_P), cap,
+                                   sub.ctypes.data_as(_P), cap // 2, ctypes.byref(ns))
+        return pts[:n].copy(), sub[:ns.value].copy()
+
+    def dash_lines(self):
+        """Reference seam: lines after path_to_lines(true) and dash_lines."""
+        lib = self.fns["lib"]
+        cap = 1 << 22
+        pts = np.zeros((cap, 2), dtype=np.float32)
+        sub = np.zeros((cap // 2, 2), dtype=np.int32)
+        ns = ctypes.c_long()
+        n = lib.cih_ref_dash_lines(self.handle, pts.ctypes.data_as(_P), cap,
                                    sub.ctypes.data_as(_P), cap // 2, ctypes.byref(ns))
         return pts[:n].copy(), sub[:ns.value].copy()
 

@@ -90,6 +90,28 @@ long cih_ref_path_lines(void *handle, int stroking, float *points, long capacity
     return n;
 }
 
+// Stage seam: flattens for stroking (hpp:1495) and applies the dash pattern
+// (hpp:1858) without stroke expansion.
+long cih_ref_dash_lines(void *handle, float *points, long capacity, int *subpath_info,
+                        long subpath_capacity, long *subpaths)
+{
+    canvas *c = static_cast<canvas *>(handle);
+    c->path_to_lines(true);
+    c->dash_lines();
+    long n = (long)c->lines.points.size();
+    for (long k = 0; k < n && k < capacity; ++k) {
+        points[2 * k + 0] = c->lines.points[(size_t)k].x;
+        points[2 * k + 1] = c->lines.points[(size_t)k].y;
+    }
+    long s = (long)c->lines.subpaths.size();
+    for (long k = 0; k < s && k < subpath_capacity; ++k) {
+        subpath_info[2 * k + 0] = (int)c->lines.subpaths[(size_t)k].count;
+        subpath_info[2 * k + 1] = c->lines.subpaths[(size_t)k].closed ? 1 : 0;
+    }
+    *subpaths = s;
+    return n;
+}
+
 // Stage seam: scan-converts the current polylines (hpp:2193) with the given
 // offset and padding; writes up to `capacity` runs as (x, y, delta) triples
 // (x and y stored as floats) and returns the number of runs.
