@@ -42,6 +42,7 @@ struct draw_state {
     // shadow scan conversion (extended canvas coordinates)
     int32_t srow0, srows;
     uint64_t srowoff, sruns;
+    int32_t slast; // last row with a shadow step (its last pixel is never painted)
     // shadow image: [left, left + width) x [top, top + height) of the
     // extended canvas, stored row-major at `image` (and `image2` ping-pong)
     int32_t left, top, width, height;
@@ -62,6 +63,12 @@ struct canvas_desc {
 struct run {
     int32_t x;
     float delta;
+};
+
+// A coverage step: the row's running sum of run deltas after pixel x.
+struct step {
+    int32_t x;
+    float sum;
 };
 
 // Arena counters (device, 64-bit): requested amounts keep counting past the
@@ -90,6 +97,7 @@ struct frame {
     uint32_t *words;
     uint64_t words_cap;
     run *runs;
+    step *steps; // same indexing and capacity as runs
     uint64_t runs_cap;
     float *shadow;
     uint64_t shadow_cap;

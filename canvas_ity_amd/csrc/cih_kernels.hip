@@ -8,13 +8,17 @@
 //   k_runs         one workgroup per draw and pass (main, shadow): clip
 //                  (hpp:2205-2238), signed-area runs (hpp:2109-2170) bucketed
 //                  per pixel row; the shadow pass also sizes its image
+//   k_coalesce     one wave per (draw, pass) and row stride: each row's runs
+//                  sorted, coalesced and summed along the row with the
+//                  reference's float arithmetic (hpp:2239-2252) -> coverage
+//                  steps (x, running sum) in x order
 //   k_shadow_*     shadow alpha raster, then the extended-box blur
 //                  (hpp:2453-2503) with one thread per row / per column
-//   k_fine         one workgroup per (canvas, row, segment): the fp32 pixels
-//                  stay in registers while every draw of the canvas is
-//                  applied in painter's order -- row prefix sums of the runs
-//                  in 64-bit fixed point (exact, order independent), paint
-//                  (hpp:2265-2377) and composite (hpp:2576-2592)
+//   k_fine         one wave per (canvas, row, 256-pixel segment): the fp32
+//                  pixels stay in registers while every draw of the canvas is
+//                  applied in painter's order -- coverage from the row's steps
+//                  (a 64-way search, then LDS for steps inside the segment),
+//                  paint (hpp:2265-2377) and composite (hpp:2576-2592)
 //   k_readback     get_image_data's unpremultiply/sRGB/dither (hpp:3363-3380)
 // None of this is a dense contraction: the pipeline is VALU/LDS and HBM work.
 #include <hip/hip_runtime.h>
@@ -26,8 +30,8 @@
 namespace cih {
 
 #define GEO_THREADS 256
-#define FINE_THREADS 128
-#define FINE_PX 2
+#define FINE_THREADS 64 // one wave: segment_sums relies on it
+#define FINE_PX 4
 #define FINE_SEGMENT (FINE_THREADS * FINE_PX)
 #define LDS_ROWS 4096
 
@@ -655,6 +659,7 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F) {
     S.srows = rows;
     S.srowoff = ro;
     S.sruns = rb;
+    S.slast = -1; // raised by k_coalesce
     // Shadow bounds (hpp:2409-2426) over the runs that survive coalescing:
     // those with a non-zero delta, plus the first run in sort order.
     int b = D.border;
@@ -693,105 +698,465 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F) {
 }
 
 // ---------------------------------------------------------------------------
-// Row-segment coverage shared by the shadow raster and the fine pass.
+// Coverage: runs coalesced into per-row steps, read back per row segment.
 //
-// A workgroup of FINE_THREADS owns FINE_SEGMENT pixels [x0, x0 + SEGMENT) of
-// one row; thread t owns FINE_PX consecutive pixels.  Coverage at pixel x is
-// |sum of run deltas at x' <= x| (hpp:2565-2575): deltas left of the segment
-// form the backdrop, those inside are added into LDS in 2^-32 fixed point and
-// prefix-summed.  Integer sums are exact and independent of the order in
-// which threads add them.
+// The reference sorts a draw's runs by (y, x, |delta|) (hpp:2172-2181,
+// hpp:2241), adds up the deltas of one pixel in that order (hpp:2242-2250)
+// and then keeps a float running sum along each row (hpp:2565-2604,
+// hpp:2430-2451): a pixel's coverage is min(|running sum after the last run
+// at or left of it|, 1).  k_coalesce repeats exactly that arithmetic for each
+// row bucket: the runs are sorted by the same key (equal magnitudes: the
+// negative delta first), each pixel's deltas are added in sorted order, and
+// the row's sum is carried from pixel to pixel one float addition at a time.
+// A row's result is a list of steps (x, running sum after x) in x order,
+// stored in the step arena at the row's run offsets and padded to the
+// bucket's size with sentinels (x = STEP_SENTINEL, sum = the bits of the
+// row's last step x, or of -1).  Pixels whose deltas are all zero give no
+// step: the reference's coalescing drops those runs, and they change no sum.
 
-struct seg_lds {
-    long long acc[FINE_SEGMENT];
-    long long wave[FINE_THREADS / 64];
-    int minx, maxx;
-};
+#define STEP_SENTINEL 0x7fffffff
+#define COALESCE_CAP 1024 // runs sorted at once in LDS
+#define COALESCE_ROWS 32  // workgroups per draw and pass (rows are strided)
 
-__device__ inline long long quantize(float delta) {
-    return __float2ll_rn(delta * 4294967296.0f);
+// Sort key: x, then |delta|, then the negative delta first.
+__device__ inline unsigned long long run_key(run r) {
+    uint32_t bits = __float_as_uint(r.delta);
+    return ((unsigned long long)(uint32_t)r.x << 32) |
+           ((unsigned long long)(bits & 0x7fffffffu) << 1) | (unsigned long long)(1u - (bits >> 31));
+}
+__device__ inline int key_x(unsigned long long k) { return (int)(uint32_t)(k >> 32); }
+__device__ inline float key_delta(unsigned long long k) {
+    uint32_t mag = (uint32_t)(k >> 1) & 0x7fffffffu;
+    return __uint_as_float(mag | ((uint32_t)(~k & 1ull) << 31));
 }
 
-__device__ inline float coverage(long long q, bool evenodd) {
-    if (evenodd) {
-        // extension: triangle wave of the winding area, period 2
-        unsigned long long a = (unsigned long long)(q < 0 ? -q : q) & ((1ull << 33) - 1);
-        if (a > (1ull << 32))
-            a = (1ull << 33) - a;
-        return smin((float)a * 2.3283064365386963e-10f, 1.0f);
+__device__ inline unsigned long long shfl_xor64(unsigned long long v, int mask) {
+    uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)v, mask, 64);
+    uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(v >> 32), mask, 64);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ inline unsigned long long wave_min64(unsigned long long v) {
+    for (int m = 32; m >= 1; m >>= 1) {
+        unsigned long long o = shfl_xor64(v, m);
+        v = o < v ? o : v;
     }
-    float s = (float)q * 2.3283064365386963e-10f;
-    return smin(fabsf(s), 1.0f);
-}
-
-__device__ inline long long wave_sum64(long long v) {
-    for (int d = 32; d >= 1; d >>= 1)
-        v += __shfl_xor(v, d, 64);
     return v;
 }
 
-// Fills q[k] with the running sum at pixel x0 + threadIdx.x*FINE_PX + k and
-// returns the row's first/last run x in rmin/rmax.  Uniform call.
-__device__ void segment_coverage(const run *runs, uint32_t begin, uint32_t end, int x0,
-                                 seg_lds &L, long long q[FINE_PX], int &rmin, int &rmax) {
-    int t = threadIdx.x;
-    long long back = 0;
-    int mn = 0x7fffffff, mx = -1;
-    for (uint32_t i = begin + t; i < end; i += FINE_THREADS) {
-        run r = runs[i];
-        long long v = quantize(r.delta);
-        mn = min(mn, r.x);
-        mx = max(mx, r.x);
-        int rel = r.x - x0;
-        if (rel < 0)
-            back += v;
-        else if (rel < FINE_SEGMENT)
-            atomicAdd((unsigned long long *)&L.acc[rel], (unsigned long long)v);
+__device__ inline int wave_min(int v) {
+    for (int m = 32; m >= 1; m >>= 1)
+        v = min(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+__device__ inline int wave_max(int v) {
+    for (int m = 32; m >= 1; m >>= 1)
+        v = max(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// Bitonic sort of one key per lane over the wave, ascending in lane order.
+__device__ inline unsigned long long wave_sort(unsigned long long key) {
+    int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            unsigned long long o = shfl_xor64(key, j);
+            bool take_min = ((lane & j) == 0) == ((lane & k) == 0);
+            key = take_min ? (o < key ? o : key) : (o > key ? o : key);
+        }
+    return key;
+}
+
+// Bitonic sort of buf[0, n), n a power of two, by a one-wave workgroup.
+__device__ void block_sort(unsigned long long *buf, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += 64) {
+                uint32_t p = i ^ j;
+                if (p > i) {
+                    unsigned long long a = buf[i], b = buf[p];
+                    if ((a > b) == ((i & k) == 0)) {
+                        buf[i] = b;
+                        buf[p] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// Turns a row's runs, fed in key order, into its steps.  All members are
+// wave-uniform except bx/bs, the step that lane (index mod 64) buffers for a
+// coalesced store.
+struct step_writer {
+    step *out;
+    uint32_t count;
+    int last_x;
+    int open_x; // pixel whose deltas may continue in the next chunk, or -1
+    float open_sum;
+    bool open_nz;
+    float sum; // the row's running sum
+    int bx;
+    float bs;
+
+    __device__ void init(step *o) {
+        out = o;
+        count = 0;
+        last_x = -1;
+        open_x = -1;
+        open_sum = 0.0f;
+        open_nz = false;
+        sum = 0.0f;
+        bx = 0;
+        bs = 0.0f;
     }
-    back = wave_sum64(back);
-    for (int d = 32; d >= 1; d >>= 1) {
-        mn = min(mn, __shfl_xor(mn, d, 64));
-        mx = max(mx, __shfl_xor(mx, d, 64));
+    __device__ void emit(int x, float s) {
+        int lane = threadIdx.x & 63;
+        if (lane == (int)(count & 63)) {
+            bx = x;
+            bs = s;
+        }
+        ++count;
+        last_x = x;
+        if ((count & 63) == 0) {
+            step v = {bx, bs};
+            out[count - 64 + lane] = v;
+        }
     }
-    if ((t & 63) == 0) {
-        L.wave[t >> 6] = back;
-        atomicMin(&L.minx, mn);
-        atomicMax(&L.maxx, mx);
+    __device__ void close_open() {
+        if (open_x >= 0 && open_nz) {
+            sum = sum + open_sum;
+            emit(open_x, sum);
+        }
+        open_x = -1;
     }
+    // One delta of pixel x (runs of one pixel arrive consecutively).
+    __device__ void feed_one(int x, float d) {
+        if (x != open_x) {
+            close_open();
+            open_x = x;
+            open_sum = d;
+            open_nz = d != 0.0f;
+        } else {
+            open_sum = open_sum + d;
+            open_nz = open_nz || d != 0.0f;
+        }
+    }
+    // cnt (1..64) keys in ascending lane order; `more`: later keys follow.
+    __device__ void chunk(unsigned long long key, int cnt, bool more) {
+        int lane = threadIdx.x & 63;
+        bool valid = lane < cnt;
+        int x = valid ? key_x(key) : -2;
+        float d = valid ? key_delta(key) : 0.0f;
+        int x_first = __shfl(x, 0, 64);
+        if (open_x >= 0 && x_first != open_x)
+            close_open();
+        bool cont = open_x >= 0; // lane 0 continues the open pixel
+        int prev_x = __shfl_up(x, 1, 64);
+        bool head = valid && (lane == 0 || x != prev_x);
+        float m = (lane == 0 && cont) ? open_sum + d : d;
+        bool nz = d != 0.0f || (lane == 0 && cont && open_nz);
+        for (int t = 1; t < 64; ++t) {
+            float nd = __shfl_down(d, t, 64);
+            int nx = __shfl_down(x, t, 64);
+            bool same = head && lane + t < cnt && nx == x;
+            if (same) {
+                m = m + nd;
+                nz = nz || nd != 0.0f;
+            }
+            if (__ballot(same) == 0)
+                break;
+        }
+        unsigned long long heads = __ballot(head);
+        int h = 63 - __clzll(heads); // the chunk's last pixel
+        open_x = -1;
+        if (more) {
+            open_x = __shfl(x, h, 64);
+            open_sum = __shfl(m, h, 64);
+            open_nz = __shfl((int)nz, h, 64) != 0;
+        }
+        unsigned long long kept = __ballot(head && nz && !(more && lane == h));
+        while (kept) {
+            int l = __ffsll((long long)kept) - 1;
+            kept &= kept - 1;
+            float ml = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), l));
+            int xl = __builtin_amdgcn_readlane(x, l);
+            sum = sum + ml;
+            emit(xl, sum);
+        }
+    }
+    // Closes the row: flushes buffered steps and pads to n with sentinels.
+    __device__ uint32_t finish(uint32_t n) {
+        close_open();
+        int lane = threadIdx.x & 63;
+        uint32_t r = count & 63;
+        if ((uint32_t)lane < r) {
+            step v = {bx, bs};
+            out[count - r + lane] = v;
+        }
+        step pad = {STEP_SENTINEL, __int_as_float(last_x)};
+        for (uint32_t i = count + lane; i < n; i += 64)
+            out[i] = pad;
+        return count;
+    }
+};
+
+// Coalesces one row bucket in[0, n) into out[0, n); returns the step count.
+__device__ uint32_t coalesce_row(const run *in, uint32_t n, step *out, unsigned long long *buf,
+                                 uint32_t &s_count) {
+    int lane = threadIdx.x & 63;
+    step_writer w;
+    w.init(out);
+    if (n <= 64) {
+        unsigned long long key = (uint32_t)lane < n ? run_key(in[lane]) : ~0ull;
+        w.chunk(wave_sort(key), (int)n, false);
+        return w.finish(n);
+    }
+    if (n <= COALESCE_CAP) {
+        uint32_t N = 128;
+        while (N < n)
+            N <<= 1;
+        for (uint32_t i = lane; i < N; i += 64)
+            buf[i] = i < n ? run_key(in[i]) : ~0ull;
+        __syncthreads();
+        block_sort(buf, N);
+        for (uint32_t c = 0; c < n; c += 64)
+            w.chunk((c + lane < n) ? buf[c + lane] : ~0ull, (int)min(64u, n - c), c + 64 < n);
+        __syncthreads();
+        return w.finish(n);
+    }
+    // Longer rows: windows of whole pixels holding at most COALESCE_CAP runs.
+    int xa = 0x7fffffff, xe = -1;
+    for (uint32_t i = lane; i < n; i += 64) {
+        int x = in[i].x;
+        xa = min(xa, x);
+        xe = max(xe, x);
+    }
+    xa = wave_min(xa);
+    xe = wave_max(xe) + 1;
+    uint32_t *hist = (uint32_t *)buf; // 1024 counters, aliasing the keys
+    while (xa < xe) {
+        for (int i = lane; i < 1024; i += 64)
+            hist[i] = 0;
+        __syncthreads();
+        for (uint32_t i = lane; i < n; i += 64) {
+            int x = in[i].x;
+            if (x >= xa && x - xa < 1024)
+                atomicAdd(&hist[x - xa], 1u);
+        }
+        __syncthreads();
+        uint32_t local = 0;
+        for (int j = 0; j < 16; ++j)
+            local += hist[16 * lane + j];
+        uint32_t incl = local;
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            uint32_t o = (uint32_t)__shfl_up((int)incl, dd, 64);
+            if (lane >= dd)
+                incl += o;
+        }
+        int take = 0;
+        uint32_t c = incl - local;
+        if (c <= COALESCE_CAP)
+            for (int j = 0; j < 16; ++j) {
+                c += hist[16 * lane + j];
+                if (c <= COALESCE_CAP)
+                    take = 16 * lane + j + 1;
+            }
+        take = wave_max(take);
+        __syncthreads(); // histogram read before the keys overwrite it
+        if (take == 0) {
+            // one pixel with more than COALESCE_CAP runs: feed them in key
+            // order by repeated selection
+            unsigned long long last = 0;
+            bool started = false;
+            for (;;) {
+                unsigned long long best = ~0ull;
+                for (uint32_t i = lane; i < n; i += 64) {
+                    run r = in[i];
+                    unsigned long long k = run_key(r);
+                    if (r.x == xa && (!started || k > last) && k < best)
+                        best = k;
+                }
+                best = wave_min64(best);
+                if (best == ~0ull)
+                    break;
+                int same = 0;
+                for (uint32_t i = lane; i < n; i += 64)
+                    same += (in[i].x == xa && run_key(in[i]) == best) ? 1 : 0;
+                for (int m = 32; m >= 1; m >>= 1)
+                    same += __shfl_xor(same, m, 64);
+                for (int k = 0; k < same; ++k)
+                    w.feed_one(xa, key_delta(best));
+                last = best;
+                started = true;
+            }
+            xa += 1;
+            continue;
+        }
+        int xb = xa + take;
+        if (lane == 0)
+            s_count = 0;
+        __syncthreads();
+        for (uint32_t i = lane; i < n; i += 64) {
+            run r = in[i];
+            if (r.x >= xa && r.x < xb)
+                buf[atomicAdd(&s_count, 1u)] = run_key(r);
+        }
+        __syncthreads();
+        uint32_t cnt = s_count;
+        if (cnt) {
+            uint32_t N = 64;
+            while (N < cnt)
+                N <<= 1;
+            for (uint32_t i = cnt + lane; i < N; i += 64)
+                buf[i] = ~0ull;
+            __syncthreads();
+            block_sort(buf, N);
+            for (uint32_t c2 = 0; c2 < cnt; c2 += 64)
+                w.chunk((c2 + lane < cnt) ? buf[c2 + lane] : ~0ull, (int)min(64u, cnt - c2),
+                        c2 + 64 < cnt);
+        }
+        __syncthreads(); // keys read before the next histogram
+        xa = xb;
+    }
+    return w.finish(n);
+}
+
+// One workgroup (one wave) per (draw, pass) and row stride.
+__global__ __launch_bounds__(64) void k_coalesce(frame F) {
+    __shared__ unsigned long long buf[COALESCE_CAP];
+    __shared__ uint32_t s_count;
+    int d = blockIdx.x >> 1;
+    int variant = blockIdx.x & 1;
+    if (d >= F.ndraws || overflowed(F))
+        return;
+    const draw_desc &D = F.draws[d];
+    draw_state &S = F.state[d];
+    if (variant && !D.shadow)
+        return;
+    int rows = variant ? S.srows : S.rows;
+    const uint32_t *off = F.words + (variant ? S.srowoff : S.rowoff);
+    uint64_t base = variant ? S.sruns : S.runs;
+    int last = -1;
+    for (int r = blockIdx.y; r < rows; r += gridDim.y) {
+        uint32_t b = off[r], e = off[r + 1];
+        if (b == e)
+            continue;
+        if (coalesce_row(F.runs + base + b, e - b, F.steps + base + b, buf, s_count))
+            last = r;
+    }
+    if (variant && last >= 0 && threadIdx.x == 0)
+        atomicMax(&S.slast, S.srow0 + last);
+}
+
+// Reading steps.  A fine workgroup is one wave owning FINE_SEGMENT pixels
+// [x0, x0 + FINE_SEGMENT) of a row; lane t owns FINE_PX consecutive pixels.
+
+struct seg_lds {
+    float sum[FINE_SEGMENT];
+    uint32_t mask[FINE_SEGMENT / 32]; // pixels with a step
+};
+
+__device__ inline float coverage(float sum, bool evenodd) {
+    if (evenodd) {
+        // extension: triangle wave of the winding sum, period 2
+        float a = fmodf(fabsf(sum), 2.0f);
+        return smin(a > 1.0f ? 2.0f - a : a, 1.0f);
+    }
+    return smin(fabsf(sum), 1.0f);
+}
+
+__device__ inline void scatter_step(seg_lds &L, step s, int x0) {
+    int p = s.x - x0;
+    L.sum[p] = s.sum;
+    atomicOr(&L.mask[p >> 5], 1u << (p & 31));
+}
+
+// Running sums of this lane's pixels in the segment starting at x0, for a
+// row whose steps are st[0, n), n > 0.  `before` gets the sum left of the
+// segment.  Returns false (uniformly) when no step lies inside the segment:
+// then every pixel's sum is `before` and sums[] is not written.  L.mask is
+// all zero on entry and on return.
+__device__ bool segment_sums(const step *st, uint32_t n, int x0, seg_lds &L, float &before,
+                             float sums[FINE_PX]) {
+    int lane = threadIdx.x & 63;
+    int x1 = x0 + FINE_SEGMENT;
+    // first step with x >= x0: 64-way search down to one wave-wide window
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 64) {
+        uint32_t stride = (hi - lo + 63) / 64;
+        uint32_t p = lo + (uint32_t)lane * stride;
+        bool lt = p < hi && st[p].x < x0;
+        int c = __popcll(__ballot(lt));
+        if (c == 0) {
+            hi = lo;
+            break;
+        }
+        uint32_t nhi = lo + (uint32_t)c * stride;
+        lo = lo + (uint32_t)(c - 1) * stride + 1;
+        hi = nhi < hi ? nhi : hi;
+    }
+    bool valid = lo + (uint32_t)lane < hi;
+    step s = {STEP_SENTINEL, 0.0f};
+    if (valid)
+        s = st[lo + lane];
+    bool lt = valid && s.x < x0;
+    int c = __popcll(__ballot(lt));
+    uint32_t first = lo + (uint32_t)c;
+    float left = __shfl(s.sum, c > 0 ? c - 1 : 0, 64);
+    before = first == 0 ? 0.0f : (c > 0 ? left : st[first - 1].sum);
+    bool in = valid && !lt && s.x < x1;
+    if (in)
+        scatter_step(L, s, x0);
+    unsigned long long m = __ballot(in);
+    bool any = m != 0;
+    uint32_t p = hi;
+    bool go = __popcll(m) == (int)(hi - lo) - c && p < n;
+    while (go) {
+        bool v2 = p + (uint32_t)lane < n;
+        step s2 = {STEP_SENTINEL, 0.0f};
+        if (v2)
+            s2 = st[p + lane];
+        bool in2 = v2 && s2.x < x1;
+        if (in2)
+            scatter_step(L, s2, x0);
+        unsigned long long m2 = __ballot(in2);
+        any = any || m2 != 0;
+        go = __popcll(m2) == 64 && p + 64 < n;
+        p += 64;
+    }
+    if (!any)
+        return false;
     __syncthreads();
-    long long backdrop = 0;
-    for (int w = 0; w < FINE_THREADS / 64; ++w)
-        backdrop += L.wave[w];
-    rmin = L.minx;
-    rmax = L.maxx;
-    long long local = 0;
+    uint32_t mw[FINE_SEGMENT / 32];
+#pragma unroll
+    for (int k = 0; k < FINE_SEGMENT / 32; ++k)
+        mw[k] = L.mask[k];
+    int base = lane * FINE_PX, wi = base >> 5;
+    int prev = -1;
+    uint32_t mine = 0;
+#pragma unroll
+    for (int k = 0; k < FINE_SEGMENT / 32; ++k) {
+        if (k < wi && mw[k])
+            prev = k * 32 + 31 - __builtin_clz(mw[k]);
+        if (k == wi)
+            mine = mw[k];
+    }
+#pragma unroll
     for (int k = 0; k < FINE_PX; ++k) {
-        local += L.acc[t * FINE_PX + k];
-        q[k] = local;
-        L.acc[t * FINE_PX + k] = 0;
-    }
-    // exclusive scan of the per-thread totals
-    int lane = t & 63, wave = t >> 6;
-    long long incl = local;
-    for (int d = 1; d < 64; d <<= 1) {
-        long long n = __shfl_up(incl, d, 64);
-        if (lane >= d)
-            incl += n;
-    }
-    __syncthreads(); // everyone has read L.wave / minx / maxx
-    if (lane == 63)
-        L.wave[wave] = incl;
-    if (t == 0) {
-        L.minx = 0x7fffffff;
-        L.maxx = -1;
+        int bit = (base + k) & 31;
+        uint32_t mk = mine & (0xffffffffu >> (31 - bit));
+        int pos = mk ? wi * 32 + 31 - __builtin_clz(mk) : prev;
+        sums[k] = pos >= 0 ? L.sum[pos] : before;
     }
     __syncthreads();
-    long long before = backdrop + incl - local;
-    for (int w = 0; w < wave; ++w)
-        before += L.wave[w];
-    for (int k = 0; k < FINE_PX; ++k)
-        q[k] += before;
-    __syncthreads(); // L.wave free for the next call
+    if (lane < FINE_SEGMENT / 32)
+        L.mask[lane] = 0;
+    __syncthreads();
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -948,6 +1313,9 @@ __device__ inline float4 scaled(float4 c, float s) {
 // Shadows.
 
 // Alpha raster of the shadow pass over its padded bounding box (hpp:2427-2452).
+// A row is painted from its first step through its last step's pixel, except
+// that the reference never paints the pixel of the last run of all (its loop
+// paints a run's pixels only when the next run arrives).
 __global__ __launch_bounds__(FINE_THREADS) void k_shadow_raster(frame F) {
     __shared__ seg_lds L;
     int d = blockIdx.x;
@@ -957,33 +1325,41 @@ __global__ __launch_bounds__(FINE_THREADS) void k_shadow_raster(frame F) {
     if (!D.shadow)
         return;
     const draw_state &S = F.state[d];
-    for (int i = threadIdx.x; i < FINE_SEGMENT; i += FINE_THREADS)
-        L.acc[i] = 0;
-    if (threadIdx.x == 0) {
-        L.minx = 0x7fffffff;
-        L.maxx = -1;
-    }
+    if (threadIdx.x < FINE_SEGMENT / 32)
+        L.mask[threadIdx.x] = 0;
     __syncthreads();
     float *image = F.shadow + S.image;
-    const run *runs = F.runs + S.sruns;
+    const step *steps = F.steps + S.sruns;
     const uint32_t *rowoff = F.words + S.srowoff;
+    bool evenodd = (D.flags & CIH_DRAW_FLAG_EVENODD) != 0;
     f2 offset = mk((float)D.border + D.shadow_offset_x, (float)D.border + D.shadow_offset_y);
     for (int r = blockIdx.y; r < S.height; r += gridDim.y) {
         int y = S.top + r;
         bool hit = y >= S.srow0 && y < S.srow0 + S.srows;
         uint32_t begin = hit ? rowoff[y - S.srow0] : 0, end = hit ? rowoff[y - S.srow0 + 1] : 0;
+        int first = -1, stop = -2;
+        if (begin != end) {
+            step a = steps[begin], z = steps[end - 1];
+            if (a.x != STEP_SENTINEL) {
+                first = a.x;
+                stop = z.x != STEP_SENTINEL ? z.x : __float_as_int(z.sum);
+                if (y == S.slast)
+                    --stop;
+            }
+        }
         for (int x0 = S.left; x0 < S.left + S.width; x0 += FINE_SEGMENT) {
-            long long q[FINE_PX] = {0};
-            int rmin = 0x7fffffff, rmax = -1;
-            if (begin != end)
-                segment_coverage(runs, begin, end, x0, L, q, rmin, rmax);
+            float before = 0.0f, sums[FINE_PX];
+            bool seg_hit = first >= 0 && first < x0 + FINE_SEGMENT && stop >= x0 && first <= stop;
+            bool inside = false;
+            if (seg_hit)
+                inside = segment_sums(steps + begin, end - begin, x0, L, before, sums);
             for (int k = 0; k < FINE_PX; ++k) {
                 int x = x0 + threadIdx.x * FINE_PX + k;
                 if (x >= S.left + S.width)
                     continue;
                 float alpha = 0.0f;
-                if (x >= rmin && x <= rmax) {
-                    float cov = coverage(q[k], (D.flags & CIH_DRAW_FLAG_EVENODD) != 0);
+                if (seg_hit && x >= first && x <= stop) {
+                    float cov = coverage(inside ? sums[k] : before, evenodd);
                     if (cov >= kThreshold) {
                         f2 p = sub(mk((float)x + 0.5f, (float)y + 0.5f), offset);
                         alpha = cov * paint(F, D, p).w;
@@ -1074,12 +1450,8 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
     int y = (int)(local / (uint32_t)segs);
     int x0 = (int)(local % (uint32_t)segs) * FINE_SEGMENT;
     float4 *bitmap = (float4 *)(uintptr_t)C.bitmap + (size_t)y * C.width;
-    for (int i = threadIdx.x; i < FINE_SEGMENT; i += FINE_THREADS)
-        L.acc[i] = 0;
-    if (threadIdx.x == 0) {
-        L.minx = 0x7fffffff;
-        L.maxx = -1;
-    }
+    if (threadIdx.x < FINE_SEGMENT / 32)
+        L.mask[threadIdx.x] = 0;
     float4 back[FINE_PX];
     int xs[FINE_PX];
     for (int k = 0; k < FINE_PX; ++k) {
@@ -1111,26 +1483,36 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
                 }
             }
         }
-        bool clear = !(op & 8); // whole-canvas operators (hpp:2574)
+        // Pixels of the row get |running sum| as coverage up to the canvas
+        // edge (the clip mask's last run, hpp:2636-2642, ends each row);
+        // operators without bit 8 repaint every pixel (hpp:2574).
+        bool clear = !(op & 8);
         bool rows_hit = S.rows > 0 && y >= S.row0 && y < S.row0 + S.rows;
         if (!rows_hit && !clear)
             continue;
-        long long q[FINE_PX] = {0};
-        int rmin = 0x7fffffff, rmax = -1;
+        float before = 0.0f, sums[FINE_PX];
+        bool inside = false;
         if (rows_hit) {
             const uint32_t *rowoff = F.words + S.rowoff;
             uint32_t begin = rowoff[y - S.row0], end = rowoff[y - S.row0 + 1];
             if (begin == end && !clear)
                 continue;
             if (begin != end)
-                segment_coverage(F.runs + S.runs, begin, end, x0, L, q, rmin, rmax);
+                inside = segment_sums(F.steps + S.runs + begin, end - begin, x0, L, before,
+                                      sums);
         }
         bool evenodd = (D.flags & CIH_DRAW_FLAG_EVENODD) != 0;
+        if (!inside) {
+            if (!clear && !(coverage(before, evenodd) >= kThreshold))
+                continue;
+            for (int k = 0; k < FINE_PX; ++k)
+                sums[k] = before;
+        }
         for (int k = 0; k < FINE_PX; ++k) {
             int x = xs[k];
-            if (x >= C.width || (!clear && (x < rmin || x > rmax)))
+            if (x >= C.width)
                 continue;
-            float cov = coverage(q[k], evenodd);
+            float cov = coverage(sums[k], evenodd);
             if (cov >= kThreshold || clear) {
                 float4 p = paint(F, D, mk((float)x + 0.5f, (float)y + 0.5f));
                 composite(back[k], scaled(p, cov * D.global_alpha), op, 1.0f);
@@ -1141,7 +1523,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
     unsigned long long total = composites;
     for (int w = 32; w >= 1; w >>= 1)
         total += __shfl_xor(total, w, 64);
-    if ((threadIdx.x & 63) == 0 && total)
+    if (threadIdx.x == 0 && total)
         atomicAdd(&F.counters[COUNTER_PIXEL_DRAWS], total);
     for (int k = 0; k < FINE_PX; ++k)
         if (xs[k] < C.width)
@@ -1234,10 +1616,10 @@ namespace cih {
 
 int fine_segment() { return FINE_SEGMENT; }
 
-const int kFrameKernels = 7;
+const int kFrameKernels = 8;
 extern const char *const kFrameKernelNames[kFrameKernels] = {
-    "k_reset", "k_geometry", "k_runs", "k_shadow_raster", "k_blur_rows", "k_blur_cols",
-    "k_fine"};
+    "k_reset", "k_geometry", "k_runs", "k_coalesce", "k_shadow_raster", "k_blur_rows",
+    "k_blur_cols", "k_fine"};
 
 // Launches a frame's kernels on `stream`.  With `events` (kFrameKernels + 1
 // of them), event k is recorded before kernel k and the last after the last
@@ -1258,19 +1640,22 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
     if (F.ndraws > 0)
         k_runs<<<2 * F.ndraws, GEO_THREADS, 0, stream>>>(F);
     MARK(3);
+    if (F.ndraws > 0)
+        k_coalesce<<<dim3(2 * F.ndraws, COALESCE_ROWS), 64, 0, stream>>>(F);
+    MARK(4);
     int ry = (shadow_rows + 63) / 64, cx = (shadow_cols + 63) / 64;
     if (F.ndraws > 0 && any_shadow)
         k_shadow_raster<<<dim3(F.ndraws, 64), FINE_THREADS, 0, stream>>>(F);
-    MARK(4);
-    if (F.ndraws > 0 && any_shadow)
-        k_blur_rows<<<dim3(F.ndraws, ry > 0 ? ry : 1), 64, 0, stream>>>(F);
     MARK(5);
     if (F.ndraws > 0 && any_shadow)
-        k_blur_cols<<<dim3(F.ndraws, cx > 0 ? cx : 1), 64, 0, stream>>>(F);
+        k_blur_rows<<<dim3(F.ndraws, ry > 0 ? ry : 1), 64, 0, stream>>>(F);
     MARK(6);
+    if (F.ndraws > 0 && any_shadow)
+        k_blur_cols<<<dim3(F.ndraws, cx > 0 ? cx : 1), 64, 0, stream>>>(F);
+    MARK(7);
     if (F.nitems)
         k_fine<<<F.nitems, FINE_THREADS, 0, stream>>>(F);
-    MARK(7);
+    MARK(8);
 #undef MARK
     return hipGetLastError();
 }

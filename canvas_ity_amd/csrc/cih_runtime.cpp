@@ -37,7 +37,7 @@ hipError_t launch_readback_batch(const void *canvases, const void *pixel_prefix,
                                  uint64_t total, void *out, hipStream_t stream);
 hipError_t launch_clear_batch(const void *canvases, const void *pixel_prefix, int ncanvases,
                               uint64_t total, hipStream_t stream);
-const int kFrameKernels = 7; // reset, geometry, runs, shadow raster, blur rows/cols, fine
+const int kFrameKernels = 8; // reset, geometry, runs, coalesce, shadow raster, blur rows/cols, fine
 extern const char *const kFrameKernelNames[kFrameKernels];
 hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool any_shadow,
                         hipEvent_t *events, hipStream_t stream);
@@ -160,12 +160,13 @@ struct image_entry {
 // can be in flight together.
 struct exec_lane {
     hipStream_t stream = nullptr;
-    dbuf points, words, runs, shadow, counters; // arenas and their counters
+    dbuf points, words, runs, steps, shadow, counters; // arenas and their counters
     dbuf state, canvases, items;                 // frame inputs and per-draw results
     hbuf host_counters;
     size_t last_ndraws = 0; // draws of the lane's last frame (for the trace seams)
     void release() {
-        dbuf *all[] = {&points, &words, &runs, &shadow, &counters, &state, &canvases, &items};
+        dbuf *all[] = {&points, &words, &runs, &steps, &shadow, &counters, &state, &canvases,
+                       &items};
         for (dbuf *b : all)
             b->release();
         host_counters.release();
@@ -316,7 +317,8 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
 int ensure_arenas(exec_lane &L, uint64_t pts, uint64_t words, uint64_t runs, uint64_t shadow) {
     int s;
     if ((s = L.points.reserve(8 * pts)) || (s = L.words.reserve(4 * words)) ||
-        (s = L.runs.reserve(sizeof(run) * runs)) || (s = L.shadow.reserve(4 * shadow)))
+        (s = L.runs.reserve(sizeof(run) * runs)) || (s = L.steps.reserve(sizeof(step) * runs)) ||
+        (s = L.shadow.reserve(4 * shadow)))
         return s;
     return CIH_OK;
 }
@@ -416,6 +418,7 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
         F.words = (uint32_t *)L.words.ptr;
         F.words_cap = L.words.bytes / 4;
         F.runs = (run *)L.runs.ptr;
+        F.steps = (step *)L.steps.ptr;
         F.runs_cap = L.runs.bytes / sizeof(run);
         F.shadow = (float *)L.shadow.ptr;
         F.shadow_cap = L.shadow.bytes / 4;
