@@ -77,7 +77,12 @@ def main():
                     help="draws of the workload timed on the CPU reference (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="instances for the per-core CPU figure (0: min(16, cores))")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 3 times (calibrates the HBM "
+                         "byte counters of a profiler against a copy of known size)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build renders independent canvases per GPU "
               "and has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
@@ -137,6 +142,19 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(stream, width, height, args.cpu_sample, threads)
     print(json.dumps(line))
+    return 0
+
+
+def copy_probe(mib):
+    import torch
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n, "launches": 3}))
     return 0
 
 
