@@ -286,82 +286,111 @@ CIH_HD void dash_subpath(Sink &out, const f2 *points, uint32_t count, bool close
     }
 }
 
-// One side of a stroke (hpp:1949-2058): walks points[beginning..ending] in
-// either direction in user space (through `inverse`), adding joins and, for
-// open subpaths, the end cap, as device-space points (through `forward`).
-template <class Sink>
-CIH_HD void half_stroke(Sink &out, const f2 *points, uint32_t beginning, uint32_t ending,
-                        bool closed, float line_width, float miter_limit, int join, int cap,
-                        const float *forward, const float *inverse) {
-    float half = line_width * 0.5f;
-    float ratio = miter_limit * miter_limit * half * half;
-    f2 in_direction = mk(0.0f, 0.0f);
-    float in_length = 0.0f;
+// One side of a stroke (hpp:1949-2058) is a walk over points[beginning..ending]
+// in either direction, in user space (through `inverse`), that keeps a point
+// only when it lies at least 1e-4 from the last kept one, and emits a join at
+// each kept point between its incoming and outgoing segments (and, for open
+// subpaths, a cap at the end) as device-space points (through `forward`).
+// The walk is split in two: stroke_walk decides which points are kept and
+// reports each join as the indices (previous kept point, joint, next point)
+// and the cap as (previous kept point, end point); stroke_join and
+// stroke_cap emit the points of one join or cap from those indices alone.
+// half_stroke chains them in walk order; the GPU runs the joins in parallel.
+
+// Reports rec.join(prev, at, next) for every join and rec.cap(prev, at) for
+// the end cap, in the order the reference emits them.
+template <class Rec>
+CIH_HD void stroke_walk(Rec &rec, const f2 *points, uint32_t beginning, uint32_t ending,
+                        bool closed, const float *inverse) {
+    const float epsilon = 1.0e-4f;
     f2 point = apply(inverse, points[beginning]);
+    float in_length = 0.0f;
+    uint32_t prev = beginning, at = beginning;
     uint32_t finish = beginning;
     uint32_t index = beginning;
-    const float epsilon = 1.0e-4f;
     do {
         f2 next = apply(inverse, points[index]);
-        f2 out_direction = normalized(sub(next, point));
         float out_length = len(sub(next, point));
         if (in_length != 0.0f && out_length >= epsilon) {
             if (closed && finish == beginning)
                 finish = index;
-            f2 side_in = add(point, mul(half, perp(in_direction)));
-            f2 side_out = add(point, mul(half, perp(out_direction)));
-            float turn = dot(perp(in_direction), out_direction);
-            if (fabsf(turn) < epsilon)
-                turn = 0.0f;
-            f2 offset = turn == 0.0f ? mk(0.0f, 0.0f)
-                                     : mul(half / turn, sub(out_direction, in_direction));
-            bool tight = (dot(offset, in_direction) < -in_length &&
-                          dot(offset, out_direction) > out_length);
-            if (turn > 0.0f && tight) {
-                f2 t = side_in;
-                side_in = side_out;
-                side_out = t;
-                t = in_direction;
-                in_direction = out_direction;
-                out_direction = t;
-                out.point(apply(forward, side_out));
-                out.point(apply(forward, point));
-                out.point(apply(forward, side_in));
-            }
-            if ((turn > 0.0f && !tight) ||
-                (turn != 0.0f && join == 0 && dot(offset, offset) <= ratio))
-                out.point(apply(forward, add(point, offset)));
-            else if (join == 2) {
-                float cosine = dot(in_direction, out_direction);
-                float angle = join_acos(smin(smax(cosine, -1.0f), 1.0f));
-                float alpha = 4.0f / 3.0f * join_tan(0.25f * angle);
-                out.point(apply(forward, side_in));
-                add_bezier(out, apply(forward, side_in),
-                           apply(forward, add(side_in, mul(alpha * half, in_direction))),
-                           apply(forward, sub(side_out, mul(alpha * half, out_direction))),
-                           apply(forward, side_out), -1.0f);
-            } else {
-                out.point(apply(forward, side_in));
-                out.point(apply(forward, side_out));
-            }
-            if (turn > 0.0f && tight) {
-                out.point(apply(forward, side_out));
-                out.point(apply(forward, point));
-                out.point(apply(forward, side_in));
-                f2 t = in_direction;
-                in_direction = out_direction;
-                out_direction = t;
-            }
+            rec.join(prev, at, index);
         }
         if (out_length >= epsilon) {
-            in_direction = out_direction;
             in_length = out_length;
+            prev = at;
+            at = index;
             point = next;
         }
         index = (index == ending ? beginning : ending > beginning ? index + 1 : index - 1);
     } while (index != finish);
-    if (closed || in_length == 0.0f)
-        return;
+    if (!closed && in_length != 0.0f)
+        rec.cap(prev, at);
+}
+
+// The points of the join at points[at] between the segments from
+// points[prev] and to points[next] (hpp:1966-2024).
+template <class Sink>
+CIH_HD void stroke_join(Sink &out, const f2 *points, uint32_t prev, uint32_t at, uint32_t nexti,
+                        float line_width, float miter_limit, int join, const float *forward,
+                        const float *inverse) {
+    float half = line_width * 0.5f;
+    float ratio = miter_limit * miter_limit * half * half;
+    const float epsilon = 1.0e-4f;
+    f2 before = apply(inverse, points[prev]);
+    f2 point = apply(inverse, points[at]);
+    f2 next = apply(inverse, points[nexti]);
+    f2 in_direction = normalized(sub(point, before));
+    float in_length = len(sub(point, before));
+    f2 out_direction = normalized(sub(next, point));
+    float out_length = len(sub(next, point));
+    f2 side_in = add(point, mul(half, perp(in_direction)));
+    f2 side_out = add(point, mul(half, perp(out_direction)));
+    float turn = dot(perp(in_direction), out_direction);
+    if (fabsf(turn) < epsilon)
+        turn = 0.0f;
+    f2 offset = turn == 0.0f ? mk(0.0f, 0.0f) : mul(half / turn, sub(out_direction, in_direction));
+    bool tight = (dot(offset, in_direction) < -in_length && dot(offset, out_direction) > out_length);
+    if (turn > 0.0f && tight) {
+        f2 t = side_in;
+        side_in = side_out;
+        side_out = t;
+        t = in_direction;
+        in_direction = out_direction;
+        out_direction = t;
+        out.point(apply(forward, side_out));
+        out.point(apply(forward, point));
+        out.point(apply(forward, side_in));
+    }
+    if ((turn > 0.0f && !tight) || (turn != 0.0f && join == 0 && dot(offset, offset) <= ratio))
+        out.point(apply(forward, add(point, offset)));
+    else if (join == 2) {
+        float cosine = dot(in_direction, out_direction);
+        float angle = join_acos(smin(smax(cosine, -1.0f), 1.0f));
+        float alpha = 4.0f / 3.0f * join_tan(0.25f * angle);
+        out.point(apply(forward, side_in));
+        add_bezier(out, apply(forward, side_in),
+                   apply(forward, add(side_in, mul(alpha * half, in_direction))),
+                   apply(forward, sub(side_out, mul(alpha * half, out_direction))),
+                   apply(forward, side_out), -1.0f);
+    } else {
+        out.point(apply(forward, side_in));
+        out.point(apply(forward, side_out));
+    }
+    if (turn > 0.0f && tight) {
+        out.point(apply(forward, side_out));
+        out.point(apply(forward, point));
+        out.point(apply(forward, side_in));
+    }
+}
+
+// The end cap at points[at], reached from points[prev] (hpp:2029-2057).
+template <class Sink>
+CIH_HD void stroke_cap(Sink &out, const f2 *points, uint32_t prev, uint32_t at,
+                       float line_width, int cap, const float *forward, const float *inverse) {
+    float half = line_width * 0.5f;
+    f2 point = apply(inverse, points[at]);
+    f2 in_direction = normalized(sub(point, apply(inverse, points[prev])));
     f2 ahead = mul(half, in_direction);
     f2 side = perp(ahead);
     if (cap == 0) {
@@ -382,6 +411,38 @@ CIH_HD void half_stroke(Sink &out, const f2 *points, uint32_t beginning, uint32_
                    apply(forward, add(sub(point, side), mul(alpha, ahead))),
                    apply(forward, sub(point, side)), -1.0f);
     }
+}
+
+template <class Sink>
+struct stroke_chain {
+    Sink &out;
+    const f2 *points;
+    float line_width, miter_limit;
+    int join, cap;
+    const float *forward, *inverse;
+    CIH_HD void join_at(uint32_t prev, uint32_t at, uint32_t next) {
+        stroke_join(out, points, prev, at, next, line_width, miter_limit, join, forward, inverse);
+    }
+    CIH_HD void cap_at(uint32_t prev, uint32_t at) {
+        stroke_cap(out, points, prev, at, line_width, cap, forward, inverse);
+    }
+};
+
+template <class Sink>
+struct stroke_chain_rec {
+    stroke_chain<Sink> *c;
+    CIH_HD void join(uint32_t prev, uint32_t at, uint32_t next) { c->join_at(prev, at, next); }
+    CIH_HD void cap(uint32_t prev, uint32_t at) { c->cap_at(prev, at); }
+};
+
+// One side of a stroke, emitted in walk order.
+template <class Sink>
+CIH_HD void half_stroke(Sink &out, const f2 *points, uint32_t beginning, uint32_t ending,
+                        bool closed, float line_width, float miter_limit, int join, int cap,
+                        const float *forward, const float *inverse) {
+    stroke_chain<Sink> chain = {out, points, line_width, miter_limit, join, cap, forward, inverse};
+    stroke_chain_rec<Sink> rec = {&chain};
+    stroke_walk(rec, points, beginning, ending, closed, inverse);
 }
 
 // Stroke outline of one subpath (hpp:2081-2098): one closed loop for an open

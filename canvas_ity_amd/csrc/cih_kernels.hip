@@ -270,9 +270,129 @@ __device__ line_path flatten(const frame &F, const draw_desc &D, bool stroking,
     return out;
 }
 
-// Dash (mode 0) or stroke (mode 1) every subpath of `in`, one thread each.
-__device__ line_path per_subpath(const frame &F, const draw_desc &D, line_path in, int mode,
-                                 uint32_t *lds, uint64_t *slot) {
+// Stroke expansion with one thread per join.  One thread per (subpath, side)
+// runs stroke_walk, which only decides the kept points, and records each join
+// as three point indices (a cap as two and ~0u); every join's points depend
+// on those indices alone, so the joins are then counted and written in
+// parallel, in the order the sequential walk emits them.
+struct join_recorder {
+    uint32_t *items;
+    uint32_t n;
+    __device__ void join(uint32_t prev, uint32_t at, uint32_t next) {
+        items[3 * n] = prev;
+        items[3 * n + 1] = at;
+        items[3 * n + 2] = next;
+        ++n;
+    }
+    __device__ void cap(uint32_t prev, uint32_t at) { join(prev, at, ~0u); }
+};
+
+template <class Sink>
+__device__ void stroke_item(Sink &out, const uint32_t *item, const f2 *pts, const draw_desc &D) {
+    if (item[2] == ~0u)
+        stroke_cap(out, pts, item[0], item[1], D.line_width, D.line_cap, D.forward, D.inverse);
+    else
+        stroke_join(out, pts, item[0], item[1], item[2], D.line_width, D.miter_limit,
+                    D.line_join, D.forward, D.inverse);
+}
+
+// Stroke outline of every subpath (hpp:2070-2100): per subpath, the forward
+// side, then the backward side; an open subpath gives one closed loop, a
+// closed one an outer and an inner loop.
+__device__ line_path stroke_all(const frame &F, const draw_desc &D, line_path in,
+                                uint32_t *lds, uint64_t *slot) {
+    line_path out = {0, 0, 0, 0};
+    uint32_t nsub = in.nsub;
+    // at: point offset per subpath; room: item capacity offset per subpath;
+    // items: item count per (subpath, side); ns: output subpaths per subpath
+    uint64_t w = block_alloc(F, COUNTER_WORDS, 5 * (uint64_t)(nsub + 1), F.words_cap, slot);
+    if (w == ~0ull)
+        return out;
+    uint32_t *at = F.words + w;
+    uint32_t *room = at + nsub + 1;
+    uint32_t *items = room + nsub + 1; // 2 * nsub + 1 entries
+    uint32_t *ns = items + 2 * nsub + 1;
+    for (uint32_t s = threadIdx.x; s < nsub; s += blockDim.x) {
+        uint32_t count = in.sub[2 * s];
+        at[s] = count;
+        // a side makes at most one join per visit (< 2 * count visits) and a cap
+        room[s] = count >= 2 ? 2 * (2 * count + 2) : 0;
+        ns[s] = count >= 2 ? (in.sub[2 * s + 1] ? 2u : 1u) : 0u;
+    }
+    __syncthreads();
+    scan_array(at, nsub, lds);
+    uint32_t capacity = scan_array(room, nsub, lds);
+    uint32_t nout = scan_array(ns, nsub, lds);
+    uint64_t iw = block_alloc(F, COUNTER_WORDS, 3 * (uint64_t)capacity, F.words_cap, slot);
+    if (iw == ~0ull)
+        return out;
+    uint32_t *rec = F.words + iw;
+    for (uint32_t t = threadIdx.x; t < 2 * nsub; t += blockDim.x) {
+        uint32_t s = t >> 1, side = t & 1;
+        uint32_t count = in.sub[2 * s];
+        join_recorder r = {rec + 3 * (room[s] + side * (2 * count + 2)), 0};
+        if (count >= 2)
+            stroke_walk(r, in.pts + at[s], side ? count - 1 : 0, side ? 0 : count - 1,
+                        in.sub[2 * s + 1] != 0, D.inverse);
+        items[t] = r.n;
+    }
+    __syncthreads();
+    uint32_t total = scan_array(items, 2 * nsub, lds);
+    uint64_t cw = block_alloc(F, COUNTER_WORDS, (uint64_t)total + 1, F.words_cap, slot);
+    if (cw == ~0ull)
+        return out;
+    uint32_t *cnt = F.words + cw;
+    for (uint32_t g = threadIdx.x; g < total; g += blockDim.x) {
+        uint32_t t = find_prefix(items, 2 * nsub, g);
+        uint32_t s = t >> 1, side = t & 1;
+        uint32_t count = in.sub[2 * s];
+        const uint32_t *item = rec + 3 * (room[s] + side * (2 * count + 2) + (g - items[t]));
+        count_points c;
+        stroke_item(c, item, in.pts + at[s], D);
+        cnt[g] = c.points;
+    }
+    __syncthreads();
+    uint32_t tp = scan_array(cnt, total, lds);
+    uint64_t p = block_alloc(F, COUNTER_POINTS, tp, F.points_cap, slot);
+    uint64_t q = block_alloc(F, COUNTER_WORDS, 2 * (uint64_t)nout, F.words_cap, slot);
+    if (p == ~0ull || q == ~0ull)
+        return out;
+    f2 *pts = (f2 *)F.points + p;
+    uint32_t *sub = F.words + q;
+    for (uint32_t g = threadIdx.x; g < total; g += blockDim.x) {
+        uint32_t t = find_prefix(items, 2 * nsub, g);
+        uint32_t s = t >> 1, side = t & 1;
+        uint32_t count = in.sub[2 * s];
+        const uint32_t *item = rec + 3 * (room[s] + side * (2 * count + 2) + (g - items[t]));
+        write_flat wf = {pts, cnt[g]};
+        stroke_item(wf, item, in.pts + at[s], D);
+    }
+    for (uint32_t s = threadIdx.x; s < nsub; s += blockDim.x) {
+        if (ns[s + 1] == ns[s])
+            continue;
+        uint32_t a = cnt[items[2 * s]], b = cnt[items[2 * s + 1]], e = cnt[items[2 * s + 2]];
+        uint32_t *o = sub + 2 * ns[s];
+        if (in.sub[2 * s + 1]) {
+            o[0] = b - a;
+            o[1] = 1u;
+            o[2] = e - b;
+            o[3] = 1u;
+        } else {
+            o[0] = e - a;
+            o[1] = 1u;
+        }
+    }
+    __syncthreads();
+    out.pts = pts;
+    out.sub = sub;
+    out.npts = tp;
+    out.nsub = nout;
+    return out;
+}
+
+// Dash every subpath of `in` (hpp:1858-1934), one thread each.
+__device__ line_path dash_all(const frame &F, const draw_desc &D, line_path in, uint32_t *lds,
+                              uint64_t *slot) {
     line_path out = {0, 0, 0, 0};
     uint32_t nsub = in.nsub;
     uint64_t w = block_alloc(F, COUNTER_WORDS, 3 * (uint64_t)(nsub + 1), F.words_cap, slot);
@@ -282,9 +402,7 @@ __device__ line_path per_subpath(const frame &F, const draw_desc &D, line_path i
     uint32_t *np = at + nsub + 1;
     uint32_t *ns = np + nsub + 1;
     const float *dash = (const float *)(F.payload + D.dash_at);
-    dash_setup setup = {0, 0.0f};
-    if (mode == 0)
-        setup = dash_prepare(dash, (int)D.ndash, D.dash_offset);
+    dash_setup setup = dash_prepare(dash, (int)D.ndash, D.dash_offset);
     for (uint32_t s = threadIdx.x; s < nsub; s += blockDim.x)
         at[s] = in.sub[2 * s];
     __syncthreads();
@@ -294,11 +412,7 @@ __device__ line_path per_subpath(const frame &F, const draw_desc &D, line_path i
         const f2 *pts = in.pts + at[s];
         uint32_t count = in.sub[2 * s];
         bool closed = in.sub[2 * s + 1] != 0;
-        if (mode == 0)
-            dash_subpath(c, pts, count, closed, dash, (int)D.ndash, setup, D.inverse);
-        else
-            stroke_subpath(c, pts, count, closed, D.line_width, D.miter_limit, D.line_join,
-                           D.line_cap, D.forward, D.inverse);
+        dash_subpath(c, pts, count, closed, dash, (int)D.ndash, setup, D.inverse);
         np[s] = c.points;
         ns[s] = c.subpaths;
     }
@@ -316,11 +430,7 @@ __device__ line_path per_subpath(const frame &F, const draw_desc &D, line_path i
         const f2 *src = in.pts + at[s];
         uint32_t count = in.sub[2 * s];
         bool closed = in.sub[2 * s + 1] != 0;
-        if (mode == 0)
-            dash_subpath(wp, src, count, closed, dash, (int)D.ndash, setup, D.inverse);
-        else
-            stroke_subpath(wp, src, count, closed, D.line_width, D.miter_limit, D.line_join,
-                           D.line_cap, D.forward, D.inverse);
+        dash_subpath(wp, src, count, closed, dash, (int)D.ndash, setup, D.inverse);
         wp.finish();
     }
     __syncthreads();
@@ -359,7 +469,7 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(frame F) {
     }
     if (stroke) {
         if (D.ndash) {
-            lp = per_subpath(F, D, lp, 0, lds, slot);
+            lp = dash_all(F, D, lp, lds, slot);
             if (!lp.pts)
                 return;
             if (threadIdx.x == 0) {
@@ -369,7 +479,7 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(frame F) {
                 S.dash_nsub = lp.nsub;
             }
         }
-        lp = per_subpath(F, D, lp, 1, lds, slot);
+        lp = stroke_all(F, D, lp, lds, slot);
         if (!lp.pts)
             return;
     }
@@ -1075,61 +1185,44 @@ __device__ inline void scatter_step(seg_lds &L, step s, int x0) {
     atomicOr(&L.mask[p >> 5], 1u << (p & 31));
 }
 
-// Running sums of this lane's pixels in the segment starting at x0, for a
-// row whose steps are st[0, n), n > 0.  `before` gets the sum left of the
+// segment_sums: running sums of this lane's pixels in the segment starting at
+// x0, for a row whose steps are st[0, n), n > 0.  `before` gets the sum left of the
 // segment.  Returns false (uniformly) when no step lies inside the segment:
 // then every pixel's sum is `before` and sums[] is not written.  L.mask is
 // all zero on entry and on return.
-__device__ bool segment_sums(const step *st, uint32_t n, int x0, seg_lds &L, float &before,
-                             float sums[FINE_PX]) {
+//
+// segment_window is the second half: every step before index lo has x < x0,
+// every step from index hi on has x >= x0 (hi - lo <= 64), and lane t holds
+// st[lo + t] in s when lo + t < hi.
+__device__ bool segment_window(const step *st, uint32_t n, uint32_t lo, uint32_t hi, step s,
+                               int x0, seg_lds &L, float &before, float sums[FINE_PX]);
+
+// Scatters the steps st[p, n), 64 at a time, for as long as whole chunks lie
+// inside the segment; returns whether any was inside.
+__device__ bool scatter_rest(const step *st, uint32_t n, uint32_t p, int x0, seg_lds &L) {
     int lane = threadIdx.x & 63;
     int x1 = x0 + FINE_SEGMENT;
-    // first step with x >= x0: 64-way search down to one wave-wide window
-    uint32_t lo = 0, hi = n;
-    while (hi - lo > 64) {
-        uint32_t stride = (hi - lo + 63) / 64;
-        uint32_t p = lo + (uint32_t)lane * stride;
-        bool lt = p < hi && st[p].x < x0;
-        int c = __popcll(__ballot(lt));
-        if (c == 0) {
-            hi = lo;
-            break;
-        }
-        uint32_t nhi = lo + (uint32_t)c * stride;
-        lo = lo + (uint32_t)(c - 1) * stride + 1;
-        hi = nhi < hi ? nhi : hi;
-    }
-    bool valid = lo + (uint32_t)lane < hi;
-    step s = {STEP_SENTINEL, 0.0f};
-    if (valid)
-        s = st[lo + lane];
-    bool lt = valid && s.x < x0;
-    int c = __popcll(__ballot(lt));
-    uint32_t first = lo + (uint32_t)c;
-    float left = __shfl(s.sum, c > 0 ? c - 1 : 0, 64);
-    before = first == 0 ? 0.0f : (c > 0 ? left : st[first - 1].sum);
-    bool in = valid && !lt && s.x < x1;
-    if (in)
-        scatter_step(L, s, x0);
-    unsigned long long m = __ballot(in);
-    bool any = m != 0;
-    uint32_t p = hi;
-    bool go = __popcll(m) == (int)(hi - lo) - c && p < n;
+    bool any = false, go = p < n;
     while (go) {
-        bool v2 = p + (uint32_t)lane < n;
-        step s2 = {STEP_SENTINEL, 0.0f};
-        if (v2)
-            s2 = st[p + lane];
-        bool in2 = v2 && s2.x < x1;
-        if (in2)
-            scatter_step(L, s2, x0);
-        unsigned long long m2 = __ballot(in2);
-        any = any || m2 != 0;
-        go = __popcll(m2) == 64 && p + 64 < n;
+        bool v = p + (uint32_t)lane < n;
+        step s = {STEP_SENTINEL, 0.0f};
+        if (v)
+            s = st[p + lane];
+        bool in = v && s.x < x1;
+        if (in)
+            scatter_step(L, s, x0);
+        unsigned long long m = __ballot(in);
+        any = any || m != 0;
+        go = __popcll(m) == 64 && p + 64 < n;
         p += 64;
     }
-    if (!any)
-        return false;
+    return any;
+}
+
+// Each pixel's sum is that of the last scattered step at or left of it, or
+// `before`; leaves L.mask zero.
+__device__ void segment_lookup(seg_lds &L, float before, float sums[FINE_PX]) {
+    int lane = threadIdx.x & 63;
     __syncthreads();
     uint32_t mw[FINE_SEGMENT / 32];
 #pragma unroll
@@ -1156,7 +1249,72 @@ __device__ bool segment_sums(const step *st, uint32_t n, int x0, seg_lds &L, flo
     if (lane < FINE_SEGMENT / 32)
         L.mask[lane] = 0;
     __syncthreads();
+}
+
+__device__ bool segment_window(const step *st, uint32_t n, uint32_t lo, uint32_t hi, step s,
+                               int x0, seg_lds &L, float &before, float sums[FINE_PX]) {
+    int lane = threadIdx.x & 63;
+    int x1 = x0 + FINE_SEGMENT;
+    bool valid = lo + (uint32_t)lane < hi;
+    bool lt = valid && s.x < x0;
+    int c = __popcll(__ballot(lt));
+    uint32_t first = lo + (uint32_t)c;
+    float left = __shfl(s.sum, c > 0 ? c - 1 : 0, 64);
+    before = first == 0 ? 0.0f : (c > 0 ? left : st[first - 1].sum);
+    bool in = valid && !lt && s.x < x1;
+    if (in)
+        scatter_step(L, s, x0);
+    unsigned long long m = __ballot(in);
+    bool any = m != 0;
+    if (__popcll(m) == (int)(hi - lo) - c)
+        any = scatter_rest(st, n, hi, x0, L) || any;
+    if (!any)
+        return false;
+    segment_lookup(L, before, sums);
     return true;
+}
+
+// Like segment_window when the first step at or right of x0 is known: index
+// `first`, with lane t holding st[first + t] (a sentinel past n), and the sum
+// before the segment given.
+__device__ bool segment_fill(const step *st, uint32_t n, uint32_t first, step s, int x0,
+                             seg_lds &L, float before, float sums[FINE_PX]) {
+    int x1 = x0 + FINE_SEGMENT;
+    bool in = s.x < x1;
+    if (in)
+        scatter_step(L, s, x0);
+    unsigned long long m = __ballot(in);
+    bool any = m != 0;
+    if (__popcll(m) == 64)
+        any = scatter_rest(st, n, first + 64, x0, L) || any;
+    if (!any)
+        return false;
+    segment_lookup(L, before, sums);
+    return true;
+}
+
+__device__ bool segment_sums(const step *st, uint32_t n, int x0, seg_lds &L, float &before,
+                             float sums[FINE_PX]) {
+    int lane = threadIdx.x & 63;
+    // first step with x >= x0: 64-way search down to one wave-wide window
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 64) {
+        uint32_t stride = (hi - lo + 63) / 64;
+        uint32_t p = lo + (uint32_t)lane * stride;
+        bool lt = p < hi && st[p].x < x0;
+        int c = __popcll(__ballot(lt));
+        if (c == 0) {
+            hi = lo;
+            break;
+        }
+        uint32_t nhi = lo + (uint32_t)c * stride;
+        lo = lo + (uint32_t)(c - 1) * stride + 1;
+        hi = nhi < hi ? nhi : hi;
+    }
+    step s = {STEP_SENTINEL, 0.0f};
+    if (lo + (uint32_t)lane < hi)
+        s = st[lo + lane];
+    return segment_window(st, n, lo, hi, s, x0, L, before, sums);
 }
 
 // ---------------------------------------------------------------------------
@@ -1196,23 +1354,24 @@ __device__ float4 paint_pattern(const frame &F, const draw_desc &D, f2 point) {
     bool clamp = D.brush == CIH_BRUSH_IMAGE;
     float4 total = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float total_weight = 0.0f;
-    for (int py = top; py < bottom; ++py) {
+    // repeat wraps by the remainder (hpp:2303-2305, 2316-2318), kept
+    // incrementally over consecutive taps
+    int wy_start = top % bh, wx_start = left % bw;
+    if (wy_start < 0)
+        wy_start += bh;
+    if (wx_start < 0)
+        wx_start += bw;
+    int wrap_y = wy_start;
+    for (int py = top; py < bottom; ++py, wrap_y = wrap_y + 1 == bh ? 0 : wrap_y + 1) {
         float y = fabsf(reciprocal_y * ((float)py - point.y));
         float weight_y = cubic_weight(y);
-        int wy = py % bh;
-        if (wy < 0)
-            wy += bh;
-        if (clamp)
-            wy = min(max(py, 0), bh - 1);
+        int wy = clamp ? min(max(py, 0), bh - 1) : wrap_y;
         const float4 *row = texels + (size_t)wy * (size_t)bw;
-        for (int px = left; px < right; ++px) {
+        int wrap_x = wx_start;
+        for (int px = left; px < right; ++px, wrap_x = wrap_x + 1 == bw ? 0 : wrap_x + 1) {
             float x = fabsf(reciprocal_x * ((float)px - point.x));
             float weight_x = cubic_weight(x);
-            int wx = px % bw;
-            if (wx < 0)
-                wx += bw;
-            if (clamp)
-                wx = min(max(px, 0), bw - 1);
+            int wx = clamp ? min(max(px, 0), bw - 1) : wrap_x;
             float weight = weight_x * weight_y;
             float4 c = row[wx];
             total.x = total.x + c.x * weight;
@@ -1226,10 +1385,11 @@ __device__ float4 paint_pattern(const frame &F, const draw_desc &D, f2 point) {
     return make_float4(total.x * r, total.y * r, total.z * r, total.w * r);
 }
 
-__device__ float4 paint_gradient(const frame &F, const draw_desc &D, f2 point) {
+// Gradient colour at `point` (hpp:2331-2376); `stops` holds the ascending
+// stop offsets and `colors` four floats (linear, straight alpha) per stop.
+__device__ float4 paint_gradient(const draw_desc &D, const float *stops, const float *colors,
+                                 f2 point) {
     const float4 zero = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float *colors = (const float *)(F.payload + D.colors_at);
-    const float *stops = (const float *)(F.payload + D.stops_at);
     f2 start = mk(D.start_x, D.start_y), end = mk(D.end_x, D.end_y);
     f2 relative = sub(point, start);
     f2 line = sub(end, start);
@@ -1262,10 +1422,12 @@ __device__ float4 paint_gradient(const frame &F, const draw_desc &D, f2 point) {
         else
             return zero;
     }
+    // std::upper_bound over ascending stops = the number of stops not
+    // greater than offset (all of them for a NaN offset)
     uint32_t n = D.nstops;
-    uint32_t index = 0; // std::upper_bound
-    while (index < n && !(offset < stops[index]))
-        ++index;
+    uint32_t index = 0;
+    for (uint32_t k = 0; k < n; ++k)
+        index += offset < stops[k] ? 0u : 1u;
     if (index == 0)
         return premultiplied(load4(colors));
     if (index == n)
@@ -1277,15 +1439,22 @@ __device__ float4 paint_gradient(const frame &F, const draw_desc &D, f2 point) {
     return premultiplied(c);
 }
 
-// paint_pixel at point (already offset to the pixel centre).
-__device__ inline float4 paint(const frame &F, const draw_desc &D, f2 point) {
+// paint_pixel (hpp:2265-2377) at point (already offset to the pixel centre),
+// with the brush's stops and colours at the given addresses.
+__device__ inline float4 paint(const frame &F, const draw_desc &D, const float *stops,
+                               const float *colors, f2 point) {
     if (D.brush == CIH_BRUSH_PATTERN || D.brush == CIH_BRUSH_IMAGE)
         return paint_pattern(F, D, apply(D.paint_inverse, point));
     if (D.ncolors == 0)
         return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (D.brush == CIH_BRUSH_COLOR)
-        return load4((const float *)(F.payload + D.colors_at));
-    return paint_gradient(F, D, apply(D.paint_inverse, point));
+        return load4(colors);
+    return paint_gradient(D, stops, colors, apply(D.paint_inverse, point));
+}
+
+__device__ inline float4 paint(const frame &F, const draw_desc &D, f2 point) {
+    return paint(F, D, (const float *)(F.payload + D.stops_at),
+                 (const float *)(F.payload + D.colors_at), point);
 }
 
 __device__ inline void composite(float4 &back, float4 fore, int op, float visibility) {
@@ -1438,10 +1607,59 @@ __device__ inline uint32_t find_canvas(const uint32_t *prefix, uint32_t n, uint3
     return find_prefix(prefix, n, item);
 }
 
+// Work items go to the XCDs in contiguous ranges: workgroup b runs on XCD
+// b % FINE_XCDS (round-robin dispatch), and item = (b % FINE_XCDS) * per +
+// b / FINE_XCDS keeps the segments of one row, which read the same steps, in
+// one L2.
+#define FINE_XCDS 8
+
+__device__ inline bool fine_item(const frame &F, uint32_t &item) {
+    uint32_t per = (F.nitems + FINE_XCDS - 1) / FINE_XCDS;
+    item = (blockIdx.x % FINE_XCDS) * per + blockIdx.x / FINE_XCDS;
+    return item < F.nitems;
+}
+
+__device__ inline int lane_int(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline float lane_float(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// The draws of a canvas are taken 64 at a time: lane t reads the metadata of
+// draw d0 + t (row range, operator, brush colour) and binary-searches the
+// draw's steps in this row for the segment: the first step at or right of x0
+// and the running sum before it.  The draws with work in the segment are then
+// run in order; the next one's steps are loaded while the current one is
+// composited.
+#define STAGE_STOPS 64
+
+// A non-solid draw's descriptor, stops and colours, staged for its pixels.
+struct paint_lds {
+    uint32_t words[sizeof(draw_desc) / 4];
+    float stops[STAGE_STOPS];
+    float colors[4 * STAGE_STOPS];
+};
+static_assert(sizeof(draw_desc) % 4 == 0, "draw_desc is staged word by word");
+
+__device__ inline void paint_pixels(const frame &F, const draw_desc &D, const float *stops,
+                                    const float *colors, float4 back[FINE_PX],
+                                    const int xs[FINE_PX], int width, int y,
+                                    const float sums[FINE_PX], bool evenodd, bool clear,
+                                    float galpha, int op, unsigned &composites) {
+    for (int k = 0; k < FINE_PX; ++k) {
+        float cov = coverage(sums[k], evenodd);
+        if (xs[k] < width && (cov >= kThreshold || clear)) {
+            float4 p = paint(F, D, stops, colors, mk((float)xs[k] + 0.5f, (float)y + 0.5f));
+            composite(back[k], scaled(p, cov * galpha), op, 1.0f);
+            ++composites;
+        }
+    }
+}
+
 __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
     __shared__ seg_lds L;
-    uint32_t item = blockIdx.x;
-    if (item >= F.nitems || overflowed(F))
+    __shared__ paint_lds P;
+    uint32_t item;
+    if (!fine_item(F, item) || overflowed(F))
         return;
     uint32_t c = find_canvas(F.items, (uint32_t)F.ncanvases, item);
     const canvas_desc &C = F.canvases[c];
@@ -1449,81 +1667,178 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
     int segs = (C.width + FINE_SEGMENT - 1) / FINE_SEGMENT;
     int y = (int)(local / (uint32_t)segs);
     int x0 = (int)(local % (uint32_t)segs) * FINE_SEGMENT;
+    int x1 = x0 + FINE_SEGMENT;
+    int lane = threadIdx.x;
     float4 *bitmap = (float4 *)(uintptr_t)C.bitmap + (size_t)y * C.width;
-    if (threadIdx.x < FINE_SEGMENT / 32)
-        L.mask[threadIdx.x] = 0;
+    if (lane < FINE_SEGMENT / 32)
+        L.mask[lane] = 0;
     float4 back[FINE_PX];
     int xs[FINE_PX];
     for (int k = 0; k < FINE_PX; ++k) {
-        xs[k] = x0 + threadIdx.x * FINE_PX + k;
+        xs[k] = x0 + lane * FINE_PX + k;
         back[k] = xs[k] < C.width ? bitmap[xs[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     __syncthreads();
     unsigned composites = 0;
-    for (int d = C.first_draw; d < C.first_draw + C.ndraws; ++d) {
-        const draw_desc &D = F.draws[d];
-        const draw_state &S = F.state[d];
-        int op = D.op;
-        if (D.shadow && S.width > 0) {
-            // shadow layer (hpp:2504-2538): rows top <= y + border < bottom,
-            // columns [left - border, right - border) within the canvas
-            int sy = y + D.border - S.top;
-            if (sy >= 0 && sy < S.height) {
-                const float *row = F.shadow + S.image + (size_t)sy * S.width;
+    const step sentinel = {STEP_SENTINEL, 0.0f};
+    int dend = C.first_draw + C.ndraws;
+    for (int d0 = C.first_draw; d0 < dend; d0 += 64) {
+        int dl = d0 + lane;
+        bool need_shadow = false, need_main = false, has_steps = false;
+        const step *st = nullptr;
+        uint32_t count = 0, first = 0;
+        float before = 0.0f;
+        int op = 0, flags = 0, brush = 0;
+        float alpha = 0.0f;
+        float4 color = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (dl < dend) {
+            const draw_desc &D = F.draws[dl];
+            const draw_state &S = F.state[dl];
+            op = D.op;
+            flags = D.flags;
+            brush = D.brush;
+            alpha = D.global_alpha;
+            if (D.shadow && S.width > 0) {
+                // shadow layer (hpp:2504-2538): rows top <= y + border < bottom,
+                // columns [left - border, right - border) within the canvas
+                int sy = y + D.border - S.top;
                 int lo = max(0, S.left - D.border);
                 int hi = min(C.width, S.left + S.width - D.border);
-                float4 color = load4(D.shadow_color);
+                need_shadow = sy >= 0 && sy < S.height && lo < hi && lo < x1 && hi > x0;
+            }
+            if (S.rows > 0 && y >= S.row0 && y < S.row0 + S.rows) {
+                const uint32_t *rowoff = F.words + S.rowoff + (y - S.row0);
+                uint32_t begin = rowoff[0];
+                count = rowoff[1] - begin;
+                st = F.steps + S.runs + begin;
+            }
+            if (count) {
+                uint32_t lo = 0, hi = count;
+                while (lo < hi) {
+                    uint32_t mid = (lo + hi) >> 1;
+                    if (st[mid].x < x0)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                first = lo;
+                has_steps = lo < count && st[lo].x < x1;
+                before = lo > 0 ? st[lo - 1].sum : 0.0f;
+            }
+            // Pixels of the row get |running sum| as coverage up to the
+            // canvas edge (the clip mask's last run, hpp:2636-2642, ends each
+            // row); operators without bit 8 repaint every pixel (hpp:2574).
+            need_main = !(op & 8) || has_steps ||
+                        coverage(before, (flags & CIH_DRAW_FLAG_EVENODD) != 0) >= kThreshold;
+            if (D.brush == CIH_BRUSH_COLOR && D.ncolors)
+                color = load4((const float *)(F.payload + D.colors_at));
+        }
+        unsigned long long todo = __ballot(need_shadow || need_main);
+        unsigned long long stepped = __ballot(has_steps);
+        auto fetch = [&](unsigned long long rest) {
+            step s = sentinel;
+            if (rest) {
+                int j = __ffsll((long long)rest) - 1;
+                if ((stepped >> j) & 1) {
+                    const step *sj = (const step *)(((uint64_t)(uint32_t)lane_int(
+                                                          (int)((uintptr_t)st >> 32), j) << 32) |
+                                                     (uint32_t)lane_int((int)(uintptr_t)st, j));
+                    uint32_t at = (uint32_t)lane_int((int)first, j) + lane;
+                    if (at < (uint32_t)lane_int((int)count, j))
+                        s = sj[at];
+                }
+            }
+            return s;
+        };
+        step next = fetch(todo);
+        while (todo) {
+            int j = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            step cur = next;
+            next = fetch(todo);
+            int d = d0 + j;
+            int dop = lane_int(op, j);
+            if (lane_int((int)need_shadow, j)) {
+                const draw_desc &D = F.draws[d];
+                const draw_state &S = F.state[d];
+                const float *row =
+                    F.shadow + S.image + (size_t)(y + D.border - S.top) * S.width;
+                int lo = max(0, S.left - D.border);
+                int hi = min(C.width, S.left + S.width - D.border);
+                float4 scolor = load4(D.shadow_color);
                 for (int k = 0; k < FINE_PX; ++k) {
                     int x = xs[k];
                     if (x < lo || x >= hi)
                         continue;
-                    float s = row[x + D.border - S.left];
-                    composite(back[k], scaled(color, D.global_alpha * s), op, 1.0f);
+                    float sv = row[x + D.border - S.left];
+                    composite(back[k], scaled(scolor, D.global_alpha * sv), dop, 1.0f);
                     ++composites;
                 }
             }
-        }
-        // Pixels of the row get |running sum| as coverage up to the canvas
-        // edge (the clip mask's last run, hpp:2636-2642, ends each row);
-        // operators without bit 8 repaint every pixel (hpp:2574).
-        bool clear = !(op & 8);
-        bool rows_hit = S.rows > 0 && y >= S.row0 && y < S.row0 + S.rows;
-        if (!rows_hit && !clear)
-            continue;
-        float before = 0.0f, sums[FINE_PX];
-        bool inside = false;
-        if (rows_hit) {
-            const uint32_t *rowoff = F.words + S.rowoff;
-            uint32_t begin = rowoff[y - S.row0], end = rowoff[y - S.row0 + 1];
-            if (begin == end && !clear)
+            if (!lane_int((int)need_main, j))
                 continue;
-            if (begin != end)
-                inside = segment_sums(F.steps + S.runs + begin, end - begin, x0, L, before,
-                                      sums);
-        }
-        bool evenodd = (D.flags & CIH_DRAW_FLAG_EVENODD) != 0;
-        if (!inside) {
-            if (!clear && !(coverage(before, evenodd) >= kThreshold))
-                continue;
-            for (int k = 0; k < FINE_PX; ++k)
-                sums[k] = before;
-        }
-        for (int k = 0; k < FINE_PX; ++k) {
-            int x = xs[k];
-            if (x >= C.width)
-                continue;
-            float cov = coverage(sums[k], evenodd);
-            if (cov >= kThreshold || clear) {
-                float4 p = paint(F, D, mk((float)x + 0.5f, (float)y + 0.5f));
-                composite(back[k], scaled(p, cov * D.global_alpha), op, 1.0f);
-                ++composites;
+            bool clear = !(dop & 8);
+            bool evenodd = (lane_int(flags, j) & CIH_DRAW_FLAG_EVENODD) != 0;
+            float sums[FINE_PX];
+            float bj = lane_float(before, j);
+            bool inside = false;
+            if ((stepped >> j) & 1) {
+                const step *sj = (const step *)(((uint64_t)(uint32_t)lane_int(
+                                                     (int)((uintptr_t)st >> 32), j) << 32) |
+                                                (uint32_t)lane_int((int)(uintptr_t)st, j));
+                inside = segment_fill(sj, (uint32_t)lane_int((int)count, j),
+                                      (uint32_t)lane_int((int)first, j), cur, x0, L, bj, sums);
             }
+            if (!inside) {
+                if (!clear && !(coverage(bj, evenodd) >= kThreshold))
+                    continue;
+                for (int k = 0; k < FINE_PX; ++k)
+                    sums[k] = bj;
+            }
+            float galpha = lane_float(alpha, j);
+            if (lane_int(brush, j) == CIH_BRUSH_COLOR) {
+                float4 p = make_float4(lane_float(color.x, j), lane_float(color.y, j),
+                                       lane_float(color.z, j), lane_float(color.w, j));
+                for (int k = 0; k < FINE_PX; ++k) {
+                    float cov = coverage(sums[k], evenodd);
+                    if (xs[k] < C.width && (cov >= kThreshold || clear)) {
+                        composite(back[k], scaled(p, cov * galpha), dop, 1.0f);
+                        ++composites;
+                    }
+                }
+                continue;
+            }
+            // Gradients and patterns: the draw's descriptor (and a gradient's
+            // stops and colours) staged in LDS once for the segment's pixels.
+            const draw_desc *Dg = F.draws + d;
+            __syncthreads(); // the previous draw's reads of P are done
+            const uint32_t *src = (const uint32_t *)Dg;
+            for (int i = lane; i < (int)(sizeof(draw_desc) / 4); i += 64)
+                P.words[i] = src[i];
+            uint32_t nstops = Dg->nstops, ncolors = Dg->ncolors;
+            bool staged = nstops <= STAGE_STOPS && ncolors <= STAGE_STOPS;
+            const float *gstops = (const float *)(F.payload + Dg->stops_at);
+            const float *gcolors = (const float *)(F.payload + Dg->colors_at);
+            if (staged) {
+                for (uint32_t i = lane; i < nstops; i += 64)
+                    P.stops[i] = gstops[i];
+                for (uint32_t i = lane; i < 4 * ncolors; i += 64)
+                    P.colors[i] = gcolors[i];
+            }
+            __syncthreads();
+            const draw_desc &D = *(const draw_desc *)P.words;
+            if (staged)
+                paint_pixels(F, D, P.stops, P.colors, back, xs, C.width, y, sums, evenodd, clear,
+                             galpha, dop, composites);
+            else
+                paint_pixels(F, D, gstops, gcolors, back, xs, C.width, y, sums, evenodd, clear,
+                             galpha, dop, composites);
         }
     }
     unsigned long long total = composites;
     for (int w = 32; w >= 1; w >>= 1)
         total += __shfl_xor(total, w, 64);
-    if (threadIdx.x == 0 && total)
+    if (lane == 0 && total)
         atomicAdd(&F.counters[COUNTER_PIXEL_DRAWS], total);
     for (int k = 0; k < FINE_PX; ++k)
         if (xs[k] < C.width)
@@ -1654,7 +1969,7 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
         k_blur_cols<<<dim3(F.ndraws, cx > 0 ? cx : 1), 64, 0, stream>>>(F);
     MARK(7);
     if (F.nitems)
-        k_fine<<<F.nitems, FINE_THREADS, 0, stream>>>(F);
+        k_fine<<<FINE_XCDS * ((F.nitems + FINE_XCDS - 1) / FINE_XCDS), FINE_THREADS, 0, stream>>>(F);
     MARK(8);
 #undef MARK
     return hipGetLastError();
