@@ -497,6 +497,133 @@ CIH_HD f2 clamp_point(f2 p, float width, float height) {
 // A run sink needs `void run(int x, int y, float delta)`; runs arrive row by
 // row (y monotone) and left to right within a row.  Loops are bounded by the
 // clamped coordinates; the guards only stop malformed input.
+// add_runs row by row.  The DDA's state at the start of each pixel row has a
+// closed form, so every (edge, row) pair can be emitted on its own with the
+// same float operations as the sequential walk:
+// * row k ends at next_y_k: y = corner.y + k * y_step (integer steps are
+//   exact in float), x from the initial formula for k = 0 and from
+//   (y - from.y) * slope.x + from.x after, replaced by `to` where the walk
+//   clamps it; row k starts at next_y_{k-1} (or `from`);
+// * next_x advances by exactly 1 from corner.x, so the x steps taken through
+//   row k are the integers g >= 0 with corner.x + g < the largest next_y.x so
+//   far (at most one when next_x starts at `to`);
+// * the carry resets at each row, and each piece needs only its own and the
+//   previous next_x.
+struct edge_rows {
+    bool live; // the edge makes runs (finite, |dy| >= 2e-5)
+    bool flat_x;
+    float sign, y_step;
+    f2 from, to, slope, corner, pixel, next_x0, next_y0;
+    int rows;
+};
+
+CIH_HD edge_rows edge_rows_setup(f2 from, f2 to) {
+    const float epsilon = 2.0e-5f;
+    edge_rows E;
+    E.live = false;
+    E.rows = 0;
+    bool finite = fabsf(from.x) <= 3.0e38f && fabsf(from.y) <= 3.0e38f &&
+                  fabsf(to.x) <= 3.0e38f && fabsf(to.y) <= 3.0e38f;
+    if (!finite || fabsf(to.y - from.y) < epsilon)
+        return E;
+    E.live = true;
+    E.sign = to.y > from.y ? 1.0f : -1.0f;
+    if (from.x > to.x) {
+        f2 t = from;
+        from = to;
+        to = t;
+    }
+    E.from = from;
+    E.to = to;
+    E.pixel = mk(floorf(from.x), floorf(from.y));
+    E.corner = add(E.pixel, mk(1.0f, to.y > from.y ? 1.0f : 0.0f));
+    E.slope = mk((to.x - from.x) / (to.y - from.y), (to.y - from.y) / (to.x - from.x));
+    E.flat_x = to.x - from.x < epsilon;
+    E.next_x0 = E.flat_x ? to : mk(E.corner.x, from.y + (E.corner.x - from.x) * E.slope.y);
+    E.next_y0 = mk(from.x + (E.corner.y - from.y) * E.slope.x, E.corner.y);
+    E.y_step = to.y > from.y ? 1.0f : -1.0f;
+    // the walk ends after the first row whose end reaches to.y
+    float ahead = E.y_step > 0.0f ? ceilf(to.y) - E.corner.y : E.corner.y - floorf(to.y);
+    E.rows = (ahead > 0.0f ? (int)ahead : 0) + 1;
+    return E;
+}
+
+CIH_HD f2 edge_row_end(const edge_rows &E, int k) {
+    f2 n = E.next_y0;
+    if (k > 0) {
+        n.y = E.corner.y + (float)k * E.y_step;
+        n.x = (n.y - E.from.y) * E.slope.x + E.from.x;
+    }
+    if ((E.from.y < E.to.y && E.to.y < n.y) || (E.from.y > E.to.y && E.to.y > n.y))
+        n = E.to;
+    return n;
+}
+
+// x steps taken once next_y.x has reached v.  When next_x starts at `to`
+// (|dx| < 2e-5), rounding can still put next_y.x past to.x: the walk then
+// takes one step, after which next_x.x = to.x + 1 is out of reach.
+CIH_HD int edge_steps(const edge_rows &E, float v) {
+    if (E.flat_x)
+        return E.to.x < v ? 1 : 0;
+    if (!(v > E.corner.x))
+        return 0;
+    return (int)ceilf(v) - (int)E.corner.x;
+}
+
+CIH_HD f2 edge_next_x(const edge_rows &E, int g) {
+    if (g == 0)
+        return E.next_x0;
+    float x = E.corner.x + (float)g;
+    return mk(x, (x - E.from.x) * E.slope.y + E.from.y);
+}
+
+// Runs of row k of the edge (0 <= k < rows): the x steps, then two pieces.
+struct edge_row {
+    f2 start, end;
+    int first, steps; // x steps [first, first + steps)
+};
+
+CIH_HD edge_row edge_row_at(const edge_rows &E, int k) {
+    edge_row r;
+    r.end = edge_row_end(E, k);
+    if (k == 0) {
+        r.start = E.from;
+        r.first = 0;
+        r.steps = edge_steps(E, r.end.x);
+    } else {
+        r.start = edge_row_end(E, k - 1);
+        r.first = edge_steps(E, r.start.x);
+        r.steps = edge_steps(E, r.end.x > r.start.x ? r.end.x : r.start.x) - r.first;
+    }
+    return r;
+}
+
+CIH_HD int edge_row_runs(const edge_rows &E, int k) { return edge_row_at(E, k).steps + 2; }
+
+template <class Sink>
+CIH_HD void add_runs_row(Sink &sink, const edge_rows &E, int k) {
+    edge_row r = edge_row_at(E, k);
+    f2 now = r.start;
+    float px = E.pixel.x + (float)r.first;
+    float py = E.pixel.y + (float)k * E.y_step;
+    float carry = 0.0f;
+    for (int i = 0; i < r.steps; ++i) {
+        f2 nx = edge_next_x(E, r.first + i);
+        float strip = smin(smax((nx.y - now.y) * E.y_step, 0.0f), 1.0f);
+        float mid = (nx.x + now.x) * 0.5f;
+        float area = (mid - px) * strip;
+        sink.run((int)px, (int)py, (carry + strip - area) * E.sign);
+        carry = area;
+        now = nx;
+        px += 1.0f;
+    }
+    float strip = smin(smax((r.end.y - now.y) * E.y_step, 0.0f), 1.0f);
+    float mid = (r.end.x + now.x) * 0.5f;
+    float area = (mid - px) * strip;
+    sink.run((int)px, (int)py, (carry + strip - area) * E.sign);
+    sink.run((int)(px + 1.0f), (int)py, area * E.sign);
+}
+
 template <class Sink>
 CIH_HD void add_runs(Sink &sink, f2 from, f2 to) {
     const float epsilon = 2.0e-5f;

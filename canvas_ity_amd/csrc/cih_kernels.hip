@@ -499,59 +499,22 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(frame F) {
 // malformed (non-finite) geometry can produce; counting and writing apply the
 // same test so bucket sizes always match.
 
-struct row_counter {
-    uint32_t *rows;
-    int row0, nrows;
-    int y;
-    uint32_t n;
-    __device__ void flush() {
-        if (n)
-            atomicAdd(&rows[y - row0], n);
-        n = 0;
-    }
-    __device__ void run(int x, int yy, float) {
-        if (yy - row0 < 0 || yy - row0 >= nrows || x < 0 || x > 0xffff)
-            return;
-        if (yy != y) {
-            flush();
-            y = yy;
-        }
-        ++n;
-    }
-};
-
-struct row_writer {
-    uint32_t *cursor;
-    int row0, nrows;
+// Writes one (edge, row) item's runs contiguously and keeps the statistics
+// the shadow bounds need.  Runs with x beyond 16 bits (only an extended
+// canvas wider than 65535 pixels makes them) are dropped, in the count too.
+struct piece_writer {
     struct run *out;
+    uint32_t at;
     int minx, maxx;
     int nzminx, nzmaxx, nzminy, nzmaxy;
     unsigned long long minkey;
-    int y, n;
-    int bx[8];
-    float bd[8];
-    __device__ void flush() {
-        if (!n)
-            return;
-        uint32_t at = atomicAdd(&cursor[y - row0], (uint32_t)n);
-        for (int k = 0; k < n; ++k) {
-            struct run r;
-            r.x = bx[k];
-            r.delta = bd[k];
-            out[at + k] = r;
-        }
-        n = 0;
-    }
     __device__ void run(int x, int yy, float delta) {
-        if (yy - row0 < 0 || yy - row0 >= nrows || x < 0 || x > 0xffff)
+        if (x < 0 || x > 0xffff)
             return;
-        if (yy != y || n == 8) {
-            flush();
-            y = yy;
-        }
-        bx[n] = x;
-        bd[n] = delta;
-        ++n;
+        struct run r;
+        r.x = x;
+        r.delta = delta;
+        out[at++] = r;
         minx = min(minx, x);
         maxx = max(maxx, x);
         if (delta != 0.0f) {
@@ -567,6 +530,14 @@ struct row_writer {
         minkey = key < minkey ? key : minkey;
     }
 };
+
+// Runs of row k of an edge that fit 16-bit x: the row's runs sit at the
+// consecutive pixels first_x .. first_x + steps + 1.
+__device__ inline int item_runs(const edge_rows &E, int k) {
+    edge_row r = edge_row_at(E, k);
+    int first_x = (int)(E.pixel.x + (float)r.first);
+    return min(r.steps + 2, max(0, 0x10000 - first_x));
+}
 
 // Endpoints of edge e of the draw's clipped polygons (with the pass offset).
 struct edge_source {
@@ -710,15 +681,44 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F) {
             return;
         counter = F.words + rc;
     }
-    for (int r = threadIdx.x; r < rows; r += blockDim.x)
-        counter[r] = 0;
-    __syncthreads();
+    // Work items: (edge, pixel row) pairs, the rows of each edge in walk
+    // order; thread t takes a contiguous range of items.
+    uint64_t iw = block_alloc(F, COUNTER_WORDS, (uint64_t)nedges + 1, F.words_cap, slot);
+    if (iw == ~0ull)
+        return;
+    uint32_t *erows = F.words + iw;
     for (uint32_t e = threadIdx.x; e < nedges; e += blockDim.x) {
         f2 from, to;
         src.get(e, from, to);
-        row_counter rc = {counter, row0, rows, -1, 0};
-        add_runs(rc, clamp_point(from, width, height), clamp_point(to, width, height));
-        rc.flush();
+        edge_rows E = edge_rows_setup(clamp_point(from, width, height),
+                                      clamp_point(to, width, height));
+        erows[e] = E.live ? (uint32_t)E.rows : 0u;
+    }
+    for (int r = threadIdx.x; r < rows; r += blockDim.x)
+        counter[r] = 0;
+    __syncthreads();
+    uint32_t nitems = scan_array(erows, nedges, lds);
+    uint32_t item_lo = (uint32_t)((uint64_t)nitems * threadIdx.x / blockDim.x);
+    uint32_t item_hi = (uint32_t)((uint64_t)nitems * (threadIdx.x + 1) / blockDim.x);
+    // count
+    {
+        uint32_t e = item_lo < item_hi ? find_prefix(erows, nedges, item_lo) : 0, ecur = ~0u;
+        edge_rows E;
+        for (uint32_t g = item_lo; g < item_hi; ++g) {
+            while (g >= erows[e + 1])
+                ++e;
+            if (e != ecur) {
+                f2 from, to;
+                src.get(e, from, to);
+                E = edge_rows_setup(clamp_point(from, width, height),
+                                    clamp_point(to, width, height));
+                ecur = e;
+            }
+            int k = (int)(g - erows[e]);
+            int row = (int)(E.pixel.y + (float)k * E.y_step) - row0;
+            if (row >= 0 && row < rows)
+                atomicAdd(&counter[row], (uint32_t)item_runs(E, k));
+        }
     }
     __syncthreads();
     for (int r = threadIdx.x; r < rows; r += blockDim.x)
@@ -731,20 +731,31 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F) {
     for (int r = threadIdx.x; r < rows; r += blockDim.x)
         counter[r] = rowoff[r];
     __syncthreads();
-    row_writer wr;
-    wr.cursor = counter;
-    wr.row0 = row0;
-    wr.nrows = rows;
+    piece_writer wr;
     wr.out = F.runs + rb;
     wr.minx = 0x7fffffff, wr.maxx = -1;
     wr.nzminx = 0x7fffffff, wr.nzmaxx = -1, wr.nzminy = 0x7fffffff, wr.nzmaxy = -1;
     wr.minkey = ~0ull;
-    wr.y = -1, wr.n = 0;
-    for (uint32_t e = threadIdx.x; e < nedges; e += blockDim.x) {
-        f2 from, to;
-        src.get(e, from, to);
-        add_runs(wr, clamp_point(from, width, height), clamp_point(to, width, height));
-        wr.flush();
+    {
+        uint32_t e = item_lo < item_hi ? find_prefix(erows, nedges, item_lo) : 0, ecur = ~0u;
+        edge_rows E;
+        for (uint32_t g = item_lo; g < item_hi; ++g) {
+            while (g >= erows[e + 1])
+                ++e;
+            if (e != ecur) {
+                f2 from, to;
+                src.get(e, from, to);
+                E = edge_rows_setup(clamp_point(from, width, height),
+                                    clamp_point(to, width, height));
+                ecur = e;
+            }
+            int k = (int)(g - erows[e]);
+            int row = (int)(E.pixel.y + (float)k * E.y_step) - row0;
+            if (row < 0 || row >= rows)
+                continue;
+            wr.at = atomicAdd(&counter[row], (uint32_t)item_runs(E, k));
+            add_runs_row(wr, E, k);
+        }
     }
     atomicMin(&s_b[0], wr.minx);
     atomicMax(&s_b[1], wr.maxx);
