@@ -1651,18 +1651,105 @@ struct paint_lds {
 };
 static_assert(sizeof(draw_desc) % 4 == 0, "draw_desc is staged word by word");
 
-__device__ inline void paint_pixels(const frame &F, const draw_desc &D, const float *stops,
-                                    const float *colors, float4 back[FINE_PX],
-                                    const int xs[FINE_PX], int width, int y,
-                                    const float sums[FINE_PX], bool evenodd, bool clear,
-                                    float galpha, int op, unsigned &composites) {
+// Gradient terms that do not depend on the pixel (hpp:2336-2360), evaluated
+// once per draw with the reference's float operations.
+struct gradient_setup {
+    float inv[6];
+    f2 start, line;
+    float span, initial, change, initial_change, initial_initial, a4, reciprocal;
+    bool linear, empty;
+    uint32_t n;
+};
+
+__device__ inline gradient_setup gradient_prepare(const draw_desc &D) {
+    gradient_setup G;
+    for (int k = 0; k < 6; ++k)
+        G.inv[k] = D.paint_inverse[k];
+    G.start = mk(D.start_x, D.start_y);
+    G.line = sub(mk(D.end_x, D.end_y), G.start);
+    G.span = dot(G.line, G.line);
+    G.initial = D.start_radius;
+    G.change = D.end_radius - G.initial;
+    G.initial_change = G.initial * G.change;
+    G.initial_initial = G.initial * G.initial;
+    float a = G.span - G.change * G.change;
+    G.a4 = 4.0f * a;
+    G.reciprocal = 1.0f / (2.0f * a);
+    G.linear = D.brush == CIH_BRUSH_LINEAR;
+    G.n = D.nstops;
+    G.empty = D.ncolors == 0 || G.n == 0 || (G.linear && G.span == 0.0f) ||
+              (!G.linear && G.span == 0.0f && G.change == 0.0f);
+    return G;
+}
+
+// paint_gradient without branches on the pixel: every lane runs the same
+// instructions and the cases the reference returns early from are selected
+// at the end.  index == 0 and index == n take colour 0 and n - 1 through a
+// mix of 0, which leaves the colour unchanged.
+__device__ __forceinline__ float4 gradient_pixel(const gradient_setup &G, const float *stops,
+                                        const float *colors, f2 point) {
+    f2 p = apply(G.inv, point);
+    f2 relative = sub(p, G.start);
+    float gradient = dot(relative, G.line);
+    float offset;
+    bool zero = false;
+    if (G.linear) {
+        offset = gradient / G.span;
+    } else {
+        float b = -2.0f * (gradient + G.initial_change);
+        float c = dot(relative, relative) - G.initial_initial;
+        float discriminant = b * b - G.a4 * c;
+        float root = sqrtf(discriminant);
+        float offset_1 = (-b - root) * G.reciprocal;
+        float offset_2 = (-b + root) * G.reciprocal;
+        float radius_1 = G.initial + G.change * offset_1;
+        float radius_2 = G.initial + G.change * offset_2;
+        offset = radius_2 >= 0.0f ? offset_2 : offset_1;
+        zero = discriminant < 0.0f || !(radius_2 >= 0.0f || radius_1 >= 0.0f);
+    }
+    uint32_t n = G.n, index = 0;
+    for (uint32_t k = 0; k < n; ++k)
+        index += offset < stops[k] ? 0u : 1u;
+    uint32_t i0 = index ? index - 1 : 0, i1 = index < n ? index : n - 1;
+    float mix = (offset - stops[i0]) / (stops[i1] - stops[i0]);
+    if (index == 0 || index == n)
+        mix = 0.0f;
+    float4 c0 = load4(colors + 4 * i0), c1 = load4(colors + 4 * i1);
+    float4 c = make_float4(c0.x + (c1.x - c0.x) * mix, c0.y + (c1.y - c0.y) * mix,
+                           c0.z + (c1.z - c0.z) * mix, c0.w + (c1.w - c0.w) * mix);
+    float4 r = premultiplied(c);
+    return zero ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : r;
+}
+
+// The segment's pixels under one draw, painted with a solid colour
+// (SOLID), nothing (EMPTY) or the gradient G: every lane composites all its
+// pixels and keeps the result where the pixel is inside the canvas and
+// painted (coverage over the threshold, or an operator that repaints
+// everything).
+enum { FILL_SOLID, FILL_EMPTY, FILL_GRADIENT };
+
+template <int KIND>
+__device__ __forceinline__ void composite_pixels(float4 solid, const gradient_setup &G,
+                                                 const float *stops, const float *colors,
+                                                 float4 (&back)[FINE_PX],
+                                                 const int (&xs)[FINE_PX], int width, float py,
+                                                 const float (&sums)[FINE_PX], bool evenodd,
+                                                 bool clear, float galpha, int op,
+                                                 unsigned &composites) {
+#pragma clang loop unroll(full)
     for (int k = 0; k < FINE_PX; ++k) {
         float cov = coverage(sums[k], evenodd);
-        if (xs[k] < width && (cov >= kThreshold || clear)) {
-            float4 p = paint(F, D, stops, colors, mk((float)xs[k] + 0.5f, (float)y + 0.5f));
-            composite(back[k], scaled(p, cov * galpha), op, 1.0f);
-            ++composites;
-        }
+        bool on = xs[k] < width && (cov >= kThreshold || clear);
+        float4 f = solid;
+        if (KIND == FILL_EMPTY)
+            f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (KIND == FILL_GRADIENT)
+            f = gradient_pixel(G, stops, colors, mk((float)xs[k] + 0.5f, py));
+        float4 b = back[k];
+        composite(b, scaled(f, cov * galpha), op, 1.0f);
+        float4 a = back[k];
+        back[k] = make_float4(on ? b.x : a.x, on ? b.y : a.y, on ? b.z : a.z, on ? b.w : a.w);
+        composites += on ? 1u : 0u;
     }
 }
 
@@ -1807,16 +1894,13 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
                     sums[k] = bj;
             }
             float galpha = lane_float(alpha, j);
-            if (lane_int(brush, j) == CIH_BRUSH_COLOR) {
+            int dbrush = lane_int(brush, j);
+            if (dbrush == CIH_BRUSH_COLOR) {
                 float4 p = make_float4(lane_float(color.x, j), lane_float(color.y, j),
                                        lane_float(color.z, j), lane_float(color.w, j));
-                for (int k = 0; k < FINE_PX; ++k) {
-                    float cov = coverage(sums[k], evenodd);
-                    if (xs[k] < C.width && (cov >= kThreshold || clear)) {
-                        composite(back[k], scaled(p, cov * galpha), dop, 1.0f);
-                        ++composites;
-                    }
-                }
+                gradient_setup none;
+                composite_pixels<FILL_SOLID>(p, none, nullptr, nullptr, back, xs, C.width, 0.0f,
+                                             sums, evenodd, clear, galpha, dop, composites);
                 continue;
             }
             // Gradients and patterns: the draw's descriptor (and a gradient's
@@ -1838,12 +1922,33 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
             }
             __syncthreads();
             const draw_desc &D = *(const draw_desc *)P.words;
-            if (staged)
-                paint_pixels(F, D, P.stops, P.colors, back, xs, C.width, y, sums, evenodd, clear,
-                             galpha, dop, composites);
-            else
-                paint_pixels(F, D, gstops, gcolors, back, xs, C.width, y, sums, evenodd, clear,
-                             galpha, dop, composites);
+            float py = (float)y + 0.5f;
+            if (dbrush == CIH_BRUSH_LINEAR || dbrush == CIH_BRUSH_RADIAL) {
+                gradient_setup G = gradient_prepare(D);
+                float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                if (G.empty)
+                    composite_pixels<FILL_EMPTY>(zero4, G, nullptr, nullptr, back, xs, C.width, py,
+                                                 sums, evenodd, clear, galpha, dop, composites);
+                else if (staged)
+                    composite_pixels<FILL_GRADIENT>(zero4, G, P.stops, P.colors, back, xs,
+                                                    C.width, py, sums, evenodd, clear, galpha,
+                                                    dop, composites);
+                else
+                    composite_pixels<FILL_GRADIENT>(zero4, G, gstops, gcolors, back, xs, C.width,
+                                                    py, sums, evenodd, clear, galpha, dop,
+                                                    composites);
+                continue;
+            }
+            // patterns: bicubic taps, only for the pixels that are painted
+#pragma clang loop unroll(full)
+            for (int k = 0; k < FINE_PX; ++k) {
+                float cov = coverage(sums[k], evenodd);
+                if (xs[k] < C.width && (cov >= kThreshold || clear)) {
+                    float4 p = paint(F, D, mk((float)xs[k] + 0.5f, py));
+                    composite(back[k], scaled(p, cov * galpha), dop, 1.0f);
+                    ++composites;
+                }
+            }
         }
     }
     unsigned long long total = composites;
