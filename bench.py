@@ -77,6 +77,17 @@ def main():
                     help="draws of the workload timed on the CPU reference (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="instances for the per-core CPU figure (0: min(16, cores))")
+    ap.add_argument("--batch", type=int, default=0, metavar="N",
+                    help="also measure the batch mix: N canvases rendered as one batch vs "
+                         "one canvas at a time (canvases/s); a batch holds at most 64M "
+                         "pixels (about 600 canvases of this mix)")
+    ap.add_argument("--serve", type=float, default=0.0, metavar="QPS",
+                    help="also measure the serving loop: Poisson arrivals at QPS requests/s "
+                         "of the serving mix, p50/p99 latency (0: skip)")
+    ap.add_argument("--requests", type=int, default=400,
+                    help="requests sent in the serving measurement")
+    ap.add_argument("--skip-main", action="store_true",
+                    help="skip the large-2048 line (only with --batch / --serve)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 3 times (calibrates the HBM "
                          "byte counters of a profiler against a copy of known size)")
@@ -89,6 +100,15 @@ def main():
         return 2
 
     import canvas_ity_amd as cia
+    if args.skip_main:
+        device = cia.default_device()
+        line = {}
+        if args.batch:
+            line["batch"] = bench_batch(cia, device, args.batch)
+        if args.serve:
+            line["serving"] = bench_serve(cia, device, args.serve, args.requests)
+        print(json.dumps(line))
+        return 0
     stream = cia.scene(2, SEED)
     width, height = cia.stream_size(stream)
     device = cia.default_device()
@@ -141,8 +161,103 @@ def main():
     if args.cpu_sample > 0:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(stream, width, height, args.cpu_sample, threads)
+    canvas.close()
+    if args.batch:
+        line["batch"] = bench_batch(cia, device, args.batch)
+    if args.serve:
+        line["serving"] = bench_serve(cia, device, args.serve, args.requests)
     print(json.dumps(line))
     return 0
+
+
+def bench_batch(cia, device, n, seed=0xC4, reps=3):
+    """Batch mix (canvases log-uniform 64..1024, 10..400 paths each): one
+    batch of n canvases vs the same canvases one at a time; both from draws
+    resident in HBM, sRGB8 written to device memory."""
+    streams = [cia.scene(4, cia.scene_seed(seed, i)) for i in range(n)]
+    pixels = sum(w * h for w, h in map(cia.stream_size, streams))
+    b = cia.Batch(streams, device)
+    b.execute(to_host=False)  # sizes the arenas, captures the graph
+    b.execute(to_host=False)
+    device.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.execute(to_host=False)
+    device.synchronize()
+    batched = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    b.execute(to_host=True)
+    with_host = time.perf_counter() - t0
+    b.close()
+    canvases = []
+    for s in streams:
+        w, h = cia.stream_size(s)
+        c = cia.Canvas(w, h, device)
+        c.replay(s)
+        c.flush()
+        canvases.append(c)
+    for c in canvases:
+        c.execute(clear=True, readback=True)
+    device.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for c in canvases:
+            c.execute(clear=True, readback=True)
+    device.synchronize()
+    plain = (time.perf_counter() - t0) / reps
+    for c in canvases:
+        c.close()
+    return {"workload": "batch mix: %d canvases log-uniform 64..1024, 10..400 paths (seed %#x)"
+                        % (n, seed),
+            "pixels": pixels, "batched_canvases_per_s": round(n / batched, 2),
+            "plain_canvases_per_s": round(n / plain, 2), "ratio": round(plain / batched, 3),
+            "batched_mpix_per_s": round(pixels / batched / 1e6, 2),
+            "batched_with_srgb8_to_host_canvases_per_s": round(n / with_host, 2)}
+
+
+def bench_serve(cia, device, qps, nreq, seed=0xC5, distinct=64):
+    """Serving mix (canvases log-uniform 128..2048, 50..2000 paths, shadows on
+    10% of requests): open-loop Poisson arrivals at `qps`; latency is submit
+    to sRGB8 on the host.  `distinct` seeded requests are sent in turn."""
+    import numpy as np
+    streams = [cia.scene(5, cia.scene_seed(seed, i)) for i in range(distinct)]
+    server = cia.Server(hbm_budget=64 << 30, max_batch=64, device=device)
+    for s in streams[:8]:  # warm-up
+        server.submit(s)
+    for _ in range(8):
+        server.result(server.wait(timeout_ms=120000))
+    gaps = np.random.default_rng(seed).exponential(1.0 / qps, nreq)
+    arrivals = np.cumsum(gaps)
+    latencies = []
+    finished = [0.0]
+
+    def collect():
+        for _ in range(nreq):
+            rid = server.wait(timeout_ms=300000)
+            if rid is None:
+                return
+            latencies.append(server.result(rid)[1])
+        finished[0] = time.perf_counter()
+    collector = threading.Thread(target=collect)
+    collector.start()
+    t0 = time.perf_counter()
+    for i in range(nreq):
+        delay = t0 + arrivals[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        server.submit(streams[i % distinct])
+    collector.join()
+    stats = server.stats()
+    server.stop()
+    wall = finished[0] - t0
+    lat = np.array(latencies)
+    return {"workload": "serving mix: canvases log-uniform 128..2048, 50..2000 paths, "
+                        "10%% shadowed (seed %#x, %d distinct)" % (seed, distinct),
+            "offered_qps": qps, "requests": nreq, "completed": int(lat.size),
+            "achieved_qps": round(lat.size / wall, 2) if wall > 0 else None,
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "batches": stats["batches"], "max_in_flight": stats["max_in_flight"]}
 
 
 def copy_probe(mib):

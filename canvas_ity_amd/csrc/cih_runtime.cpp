@@ -38,6 +38,12 @@ hipError_t launch_readback_batch(const void *canvases, const void *pixel_prefix,
 hipError_t launch_clear_batch(const void *canvases, const void *pixel_prefix, int ncanvases,
                               uint64_t total, hipStream_t stream);
 const int kFrameKernels = 8; // reset, geometry, runs, coalesce, shadow raster, blur rows/cols, fine
+
+// Canvas pixels one batch (and one server batch) may hold.  A batch of 4096
+// batch-mix canvases (491M pixels) has faulted the device once for a reason
+// not yet found, while batches up to 55M pixels have run cleanly; larger
+// batches are refused until the cause is known.
+const uint64_t kBatchPixelLimit = (uint64_t)64 << 20;
 extern const char *const kFrameKernelNames[kFrameKernels];
 hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool any_shadow,
                         hipEvent_t *events, hipStream_t stream);
@@ -1138,6 +1144,12 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
                 b->pixel_prefix[(size_t)i] + (uint64_t)r.width * (uint64_t)r.height;
         }
         uint64_t total = b->pixel_prefix.back();
+        if (total > kBatchPixelLimit) {
+            destroy_batch(b);
+            return fail(CIH_ERROR_BUDGET, "a batch may hold at most " +
+                                              std::to_string(kBatchPixelLimit) +
+                                              " canvas pixels in this build");
+        }
         if ((s = b->bitmaps.reserve(16 * total)) || (s = b->srgb.reserve(4 * total)) ||
             (s = b->prefix_dev.reserve(8 * b->pixel_prefix.size())) ||
             (s = b->all_canvases.reserve(sizeof(canvas_desc) * (size_t)count))) {
@@ -1744,15 +1756,19 @@ void server_scheduler(cih_server *sv) {
                 continue;
             server_batch *b = new server_batch;
             size_t draws = 0;
+            uint64_t pixels = 0;
             while (!sv->prepared.empty() && (int)b->requests.size() < sv->max_batch) {
                 server_request *r = sv->prepared.front();
-                if (!b->requests.empty() && draws + r->descs.size() > (1u << 20))
+                uint64_t rp = (uint64_t)r->width * (uint64_t)r->height;
+                if (!b->requests.empty() &&
+                    (draws + r->descs.size() > (1u << 20) || pixels + rp > kBatchPixelLimit))
                     break;
                 if (!sv->allocator.alloc(r->bytes, r->pool_at))
                     break; // wait for memory: admission by HBM footprint
                 sv->prepared.pop_front();
                 b->requests.push_back(r);
                 draws += r->descs.size();
+                pixels += rp;
             }
             if (b->requests.empty()) {
                 delete b;

@@ -200,6 +200,8 @@ CIH_API int cih_canvas_replay(cih_canvas *canvas, const void *stream, size_t byt
  * it to pinned host memory and waits.  No reference counterpart: the
  * reference renders one canvas per object (canvas_ity.hpp:194). */
 typedef struct cih_batch cih_batch;
+/* A batch holds at most 64M canvas pixels in this build (CIH_ERROR_BUDGET
+ * beyond that); a server batch admits requests up to the same total. */
 CIH_API int cih_batch_create(cih_device *device, int count, const void *const *streams,
                              const size_t *bytes, cih_batch **batch);
 CIH_API int cih_batch_execute(cih_batch *batch, int to_host);

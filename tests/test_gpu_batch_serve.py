@@ -2,9 +2,10 @@
 
 * A batch of seeded canvases of the batch mix (sizes log-uniform 64..1024,
   10..400 paths) renders each canvas exactly as the plain per-canvas path does
-  (bit-identical fp32 and sRGB8: both run the same kernels, and coverage sums
-  are exact integers, so the grouping of canvases into frames cannot change a
-  result), and within the parity bar of the CPU oracle.
+  (bit-identical fp32 and sRGB8: both run the same kernels, and each draw's
+  coverage comes from its own runs alone, so the grouping of canvases into
+  frames cannot change a result), and within the parity bar of the CPU oracle.
+* A batch above the 64M-pixel limit is refused before anything is allocated.
 * The server returns, for requests submitted together and interleaved, the
   same pixels each request gets when rendered alone; a memory budget smaller
   than the burst forces admission to wait for memory and still completes
@@ -38,6 +39,13 @@ def test_batch_matches_plain_and_oracle(device):
             ref8, refl = oracle.render(s, "port")
             assert_parity(ref8, refl, got8, gotl, "batch canvas %d" % i)
     b.close()
+
+
+def test_batch_refuses_oversized(device):
+    # 5 canvases of 4096x4096 hold 84M pixels, above the 64M-pixel batch limit
+    streams = [cia.scene(4, cia.scene_seed(0x0B, i), 4096, 4096, 10) for i in range(5)]
+    with pytest.raises(cia.CihError):
+        cia.Batch(streams, device)
 
 
 def test_batch_graph_replay_is_stable(device):
