@@ -44,6 +44,9 @@ const int kFrameKernels = 8; // reset, geometry, runs, coalesce, shadow raster, 
 // not yet found, while batches up to 55M pixels have run cleanly; larger
 // batches are refused until the cause is known.
 const uint64_t kBatchPixelLimit = (uint64_t)64 << 20;
+
+// Distinct images one device holds (the image table is allocated once).
+const size_t kMaxImages = (size_t)1 << 16;
 extern const char *const kFrameKernelNames[kFrameKernels];
 hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool any_shadow,
                         hipEvent_t *events, hipStream_t stream);
@@ -614,8 +617,13 @@ int cih_device_open(int ordinal, cih_device **device) {
     float table[256];
     for (int v = 0; v < 256; ++v)
         table[v] = linearized(v / 255.0f);
+    // The image table never moves: frames in flight on other streams (the
+    // serving lanes) hold its address while replay threads add images.
     if ((s = d->lut.reserve(sizeof(table))) ||
+        (s = d->image_table.reserve(sizeof(image_desc) * kMaxImages)) ||
         (e = hipMemcpy(d->lut.ptr, table, sizeof(table), hipMemcpyHostToDevice)) != hipSuccess) {
+        d->lut.release();
+        d->image_table.release();
         hipStreamDestroy(d->stream);
         delete d;
         return s ? s : hip_fail(e, "lut upload");
@@ -739,6 +747,9 @@ int cih_image_create(cih_device *d, const unsigned char *texels, int width, int 
             return CIH_OK;
         }
     }
+    if (d->images.size() >= kMaxImages)
+        return fail(CIH_ERROR_BUDGET, "a device holds at most " + std::to_string(kMaxImages) +
+                                          " distinct images");
     image_entry *e = new (std::nothrow) image_entry;
     if (!e)
         return fail(CIH_ERROR_OUT_OF_MEMORY, "out of host memory");
@@ -759,20 +770,22 @@ int cih_image_create(cih_device *d, const unsigned char *texels, int width, int 
         delete e;
         return hip_fail(err, "image upload");
     }
-    e->rgba.swap(dense);
+    // only the new entry is written: frames in flight read the others
+    image_desc entry;
+    entry.texels = (uint64_t)(uintptr_t)e->texels.ptr;
+    entry.width = width;
+    entry.height = height;
     int32_t id = (int32_t)d->images.size();
+    if ((err = hipMemcpyAsync((image_desc *)d->image_table.ptr + id, &entry, sizeof(entry),
+                              hipMemcpyHostToDevice, d->stream)) != hipSuccess ||
+        (err = hipStreamSynchronize(d->stream)) != hipSuccess) {
+        e->texels.release();
+        delete e;
+        return hip_fail(err, "image table update");
+    }
+    e->rgba.swap(dense);
     d->images.push_back(e);
     d->image_index.emplace(h, id);
-    std::vector<image_desc> table(d->images.size());
-    for (size_t k = 0; k < table.size(); ++k) {
-        table[k].texels = (uint64_t)(uintptr_t)d->images[k]->texels.ptr;
-        table[k].width = d->images[k]->width;
-        table[k].height = d->images[k]->height;
-    }
-    if ((s = d->image_table.reserve(sizeof(image_desc) * table.size())))
-        return s;
-    HIP_TRY(hipMemcpy(d->image_table.ptr, table.data(), sizeof(image_desc) * table.size(),
-                      hipMemcpyHostToDevice));
     *image = id;
     return CIH_OK;
 }
