@@ -229,14 +229,18 @@ def bench_serve(cia, device, qps, nreq, seed=0xC5, distinct=64):
     gaps = np.random.default_rng(seed).exponential(1.0 / qps, nreq)
     arrivals = np.cumsum(gaps)
     latencies = []
+    failed = []
     finished = [0.0]
 
     def collect():
         for _ in range(nreq):
             rid = server.wait(timeout_ms=300000)
             if rid is None:
-                return
-            latencies.append(server.result(rid)[1])
+                break
+            try:
+                latencies.append(server.result(rid)[1])
+            except cia.CihError as e:
+                failed.append(str(e))
         finished[0] = time.perf_counter()
     collector = threading.Thread(target=collect)
     collector.start()
@@ -254,9 +258,10 @@ def bench_serve(cia, device, qps, nreq, seed=0xC5, distinct=64):
     return {"workload": "serving mix: canvases log-uniform 128..2048, 50..2000 paths, "
                         "10%% shadowed (seed %#x, %d distinct)" % (seed, distinct),
             "offered_qps": qps, "requests": nreq, "completed": int(lat.size),
+            "failed": len(failed), "first_failure": failed[0] if failed else None,
             "achieved_qps": round(lat.size / wall, 2) if wall > 0 else None,
-            "p50_ms": round(float(np.percentile(lat, 50)), 3),
-            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3) if lat.size else None,
+            "p99_ms": round(float(np.percentile(lat, 99)), 3) if lat.size else None,
             "batches": stats["batches"], "max_in_flight": stats["max_in_flight"]}
 
 

@@ -1598,8 +1598,23 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
     std::vector<uint32_t> frame_items;
     size_t draw = 0;
     const size_t max_draws = 32768;
+    const uint64_t shadow_budget = (uint64_t)1 << 30; // shadow floats per frame
     while (draw < b->descs.size()) {
-        size_t last = std::min(b->descs.size(), draw + max_draws);
+        size_t last = draw;
+        uint64_t shadow = 0;
+        while (last < b->descs.size() && last - draw < max_draws) {
+            const draw_desc &dd = b->descs[last];
+            uint64_t need = 0;
+            if (dd.shadow) {
+                const server_request *r = b->requests[(size_t)dd.canvas];
+                int bo = dd.border;
+                need = 2ull * (uint64_t)(r->width + 2 * bo + 1) * (uint64_t)(r->height + 2 * bo + 1);
+            }
+            if (last > draw && shadow + need > shadow_budget)
+                break;
+            shadow += need;
+            ++last;
+        }
         server_batch::frame_rec f;
         f.first_draw = draw;
         f.ndraws = last - draw;
@@ -1700,10 +1715,12 @@ int finish_server_batch(cih_server *sv, int k) {
             (const unsigned long long *)((const char *)b->counters_host.ptr + 64 * f);
         if (c[COUNTER_OVERFLOW]) {
             overflow = true;
-            int s = ensure_arenas(L, 2 * std::max<uint64_t>(c[COUNTER_POINTS], L.points.bytes / 8),
-                                  2 * std::max<uint64_t>(c[COUNTER_WORDS], L.words.bytes / 4),
-                                  2 * std::max<uint64_t>(c[COUNTER_RUNS], L.runs.bytes / sizeof(run)),
-                                  2 * std::max<uint64_t>(c[COUNTER_SHADOW], L.shadow.bytes / 4));
+            // grow only the arenas this frame ran out of, to twice its request
+            auto grow = [](uint64_t asked, uint64_t cap) { return asked > cap ? 2 * asked : cap; };
+            int s = ensure_arenas(L, grow(c[COUNTER_POINTS], L.points.bytes / 8),
+                                  grow(c[COUNTER_WORDS], L.words.bytes / 4),
+                                  grow(c[COUNTER_RUNS], L.runs.bytes / sizeof(run)),
+                                  grow(c[COUNTER_SHADOW], L.shadow.bytes / 4));
             if (s)
                 return s;
         }
