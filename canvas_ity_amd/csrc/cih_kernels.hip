@@ -49,6 +49,11 @@ __device__ inline bool overflowed(const frame &F) {
     return __atomic_load_n(&F.counters[COUNTER_OVERFLOW], __ATOMIC_RELAXED) != 0;
 }
 
+// LDS ordering inside a workgroup of one wave: the wave issues its LDS
+// operations in order, so once they have completed every lane sees what the
+// others wrote.  Unlike __syncthreads, this leaves global loads in flight.
+__device__ inline void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
 // Bump allocation from an arena; returns ~0 (and flags overflow) when full.
 __device__ inline uint64_t arena_alloc(const frame &F, int which, uint64_t n, uint64_t cap) {
     unsigned long long at = atomicAdd(&F.counters[which], (unsigned long long)n);
@@ -1234,7 +1239,7 @@ __device__ bool scatter_rest(const step *st, uint32_t n, uint32_t p, int x0, seg
 // `before`; leaves L.mask zero.
 __device__ void segment_lookup(seg_lds &L, float before, float sums[FINE_PX]) {
     int lane = threadIdx.x & 63;
-    __syncthreads();
+    wave_lds_sync();
     uint32_t mw[FINE_SEGMENT / 32];
 #pragma unroll
     for (int k = 0; k < FINE_SEGMENT / 32; ++k)
@@ -1256,10 +1261,10 @@ __device__ void segment_lookup(seg_lds &L, float before, float sums[FINE_PX]) {
         int pos = mk ? wi * 32 + 31 - __builtin_clz(mk) : prev;
         sums[k] = pos >= 0 ? L.sum[pos] : before;
     }
-    __syncthreads();
+    wave_lds_sync();
     if (lane < FINE_SEGMENT / 32)
         L.mask[lane] = 0;
-    __syncthreads();
+    wave_lds_sync();
 }
 
 __device__ bool segment_window(const step *st, uint32_t n, uint32_t lo, uint32_t hi, step s,
@@ -1906,7 +1911,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
             // Gradients and patterns: the draw's descriptor (and a gradient's
             // stops and colours) staged in LDS once for the segment's pixels.
             const draw_desc *Dg = F.draws + d;
-            __syncthreads(); // the previous draw's reads of P are done
+            wave_lds_sync(); // the previous draw's reads of P are done
             const uint32_t *src = (const uint32_t *)Dg;
             for (int i = lane; i < (int)(sizeof(draw_desc) / 4); i += 64)
                 P.words[i] = src[i];
@@ -1920,7 +1925,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
                 for (uint32_t i = lane; i < 4 * ncolors; i += 64)
                     P.colors[i] = gcolors[i];
             }
-            __syncthreads();
+            wave_lds_sync();
             const draw_desc &D = *(const draw_desc *)P.words;
             float py = (float)y + 0.5f;
             if (dbrush == CIH_BRUSH_LINEAR || dbrush == CIH_BRUSH_RADIAL) {
