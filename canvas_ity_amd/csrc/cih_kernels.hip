@@ -25,6 +25,7 @@
 
 #include "cih_device.hpp"
 #include "cih_geometry.hpp"
+#include "cih_blur.hpp"
 #include "cih_abi.h"
 
 namespace cih {
@@ -1556,28 +1557,49 @@ __global__ __launch_bounds__(FINE_THREADS) void k_shadow_raster(frame F) {
     }
 }
 
-// One extended-box pass over n samples at stride `step` (hpp:2460-2481).
-__device__ inline void box_pass(const float *in, float *out, int n, size_t step, int radius,
-                                float weight_1, float weight_2) {
-    float running = weight_1 * (radius + 1 < n ? in[(size_t)(radius + 1) * step] : 0.0f);
-    for (int x = 0; x <= radius; ++x)
-        running += (weight_1 + weight_2) * (x < n ? in[(size_t)x * step] : 0.0f);
-    out[0] = running;
-    for (int x = 1; x < n; ++x) {
-        if (x >= radius + 1)
-            running -= weight_2 * in[(size_t)(x - radius - 1) * step];
-        if (x >= radius + 2)
-            running -= weight_1 * in[(size_t)(x - radius - 2) * step];
-        if (x + radius < n)
-            running += weight_2 * in[(size_t)(x + radius) * step];
-        if (x + radius + 1 < n)
-            running += weight_1 * in[(size_t)(x + radius + 1) * step];
-        out[(size_t)x * step] = running;
+// The blur of hpp:2453-2503: three extended-box passes along every row, then
+// three along every column, each pass a sequential running sum (box_line).
+// One thread runs one line, so a wave's lines must lie side by side in memory:
+// the column passes read the image as the raster wrote it (row-major), and the
+// row passes run on a transposed copy, made and undone by k_shadow_transpose.
+//
+//   raster -> image (rows)  --transpose-->  image2 (columns)
+//   row passes: image2 -> image -> image2 -> image (columns)
+//   image (columns)  --transpose-->  image2 (rows)
+//   column passes: image2 -> image -> image2 -> image (rows), read by k_fine
+
+// Copies each shadowed draw's image into image2 transposed: with to_columns,
+// image holds S.height rows of S.width samples; otherwise S.width rows of
+// S.height.  64x64 tiles through LDS, so both sides are coalesced.
+__global__ __launch_bounds__(256) void k_shadow_transpose(frame F, int to_columns) {
+    __shared__ float tile[64][65];
+    int d = blockIdx.x;
+    if (d >= F.ndraws || overflowed(F))
+        return;
+    const draw_desc &D = F.draws[d];
+    if (!D.shadow)
+        return;
+    const draw_state &S = F.state[d];
+    int rows = to_columns ? S.height : S.width, cols = to_columns ? S.width : S.height;
+    const float *src = F.shadow + S.image;
+    float *dst = F.shadow + S.image2;
+    int tile_cols = (cols + 63) / 64, tiles = ((rows + 63) / 64) * tile_cols;
+    int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    for (int t = blockIdx.y; t < tiles; t += gridDim.y) {
+        int r0 = (t / tile_cols) * 64, c0 = (t % tile_cols) * 64;
+        for (int i = ly; i < 64; i += 4)
+            if (r0 + i < rows && c0 + lx < cols)
+                tile[i][lx] = src[(size_t)(r0 + i) * cols + (c0 + lx)];
+        __syncthreads();
+        for (int i = ly; i < 64; i += 4)
+            if (c0 + i < cols && r0 + lx < rows)
+                dst[(size_t)(c0 + i) * rows + (r0 + lx)] = tile[lx][i];
+        __syncthreads();
     }
 }
 
-// Three passes along every row (one thread per row): image -> image2 ->
-// image -> image2.
+// Three passes along every row, on the transposed image (sample x of row r at
+// x * S.height + r): image2 -> image -> image2 -> image.
 __global__ __launch_bounds__(64) void k_blur_rows(frame F) {
     int d = blockIdx.x;
     if (d >= F.ndraws || overflowed(F))
@@ -1586,13 +1608,13 @@ __global__ __launch_bounds__(64) void k_blur_rows(frame F) {
     const draw_state &S = F.state[d];
     if (!D.shadow)
         return;
-    int r = blockIdx.y * 64 + threadIdx.x;
-    for (; r < S.height; r += gridDim.y * 64) {
-        float *a = F.shadow + S.image + (size_t)r * S.width;
-        float *b = F.shadow + S.image2 + (size_t)r * S.width;
-        box_pass(a, b, S.width, 1, D.radius, D.weight_1, D.weight_2);
-        box_pass(b, a, S.width, 1, D.radius, D.weight_1, D.weight_2);
-        box_pass(a, b, S.width, 1, D.radius, D.weight_1, D.weight_2);
+    size_t step = (size_t)S.height;
+    for (int r = blockIdx.y * 64 + threadIdx.x; r < S.height; r += gridDim.y * 64) {
+        float *a = F.shadow + S.image2 + r;
+        float *b = F.shadow + S.image + r;
+        box_line(a, b, S.width, step, D.radius, D.weight_1, D.weight_2);
+        box_line(b, a, S.width, step, D.radius, D.weight_1, D.weight_2);
+        box_line(a, b, S.width, step, D.radius, D.weight_1, D.weight_2);
     }
 }
 
@@ -1606,13 +1628,13 @@ __global__ __launch_bounds__(64) void k_blur_cols(frame F) {
     const draw_state &S = F.state[d];
     if (!D.shadow)
         return;
-    int c = blockIdx.y * 64 + threadIdx.x;
-    for (; c < S.width; c += gridDim.y * 64) {
+    size_t step = (size_t)S.width;
+    for (int c = blockIdx.y * 64 + threadIdx.x; c < S.width; c += gridDim.y * 64) {
         float *a = F.shadow + S.image + c;
         float *b = F.shadow + S.image2 + c;
-        box_pass(b, a, S.height, (size_t)S.width, D.radius, D.weight_1, D.weight_2);
-        box_pass(a, b, S.height, (size_t)S.width, D.radius, D.weight_1, D.weight_2);
-        box_pass(b, a, S.height, (size_t)S.width, D.radius, D.weight_1, D.weight_2);
+        box_line(b, a, S.height, step, D.radius, D.weight_1, D.weight_2);
+        box_line(a, b, S.height, step, D.radius, D.weight_1, D.weight_2);
+        box_line(b, a, S.height, step, D.radius, D.weight_1, D.weight_2);
     }
 }
 
@@ -2083,8 +2105,11 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
     if (F.ndraws > 0 && any_shadow)
         k_shadow_raster<<<dim3(F.ndraws, 64), FINE_THREADS, 0, stream>>>(F);
     MARK(5);
-    if (F.ndraws > 0 && any_shadow)
+    if (F.ndraws > 0 && any_shadow) { // the row passes run between two transposes
+        k_shadow_transpose<<<dim3(F.ndraws, 32), 256, 0, stream>>>(F, 1);
         k_blur_rows<<<dim3(F.ndraws, ry > 0 ? ry : 1), 64, 0, stream>>>(F);
+        k_shadow_transpose<<<dim3(F.ndraws, 32), 256, 0, stream>>>(F, 0);
+    }
     MARK(6);
     if (F.ndraws > 0 && any_shadow)
         k_blur_cols<<<dim3(F.ndraws, cx > 0 ? cx : 1), 64, 0, stream>>>(F);

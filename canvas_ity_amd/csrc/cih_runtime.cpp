@@ -1132,7 +1132,7 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
     if (!d || count <= 0 || !streams || !bytes || !out)
         return fail(CIH_ERROR_INVALID, "invalid batch arguments");
     *out = nullptr;
-    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    std::unique_lock<std::recursive_mutex> lock(d->mutex);
     int s;
     if ((s = set_device(d)))
         return s;
@@ -1179,18 +1179,27 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
             table[(size_t)i].first_draw = 0;
             table[(size_t)i].ndraws = 0;
         }
-        // host state machine: replay every stream into packets, in parallel
+        // host state machine: replay every stream into packets, in parallel.
+        // The replay threads add images to the device (which takes the device
+        // lock), so this thread lets go of it until they are done.
+        lock.unlock();
         unsigned workers = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         workers = std::min<unsigned>(workers, (unsigned)count);
         std::vector<int> status(workers, CIH_OK);
         std::vector<std::thread> pool;
+        pool.reserve(workers);
         size_t per = ((size_t)count + workers - 1) / workers;
         for (unsigned w = 0; w < workers; ++w) {
             size_t first = w * per, last = std::min((size_t)count, first + per);
-            pool.emplace_back(replay_range, b, streams, bytes, first, last, &status[w]);
+            try {
+                pool.emplace_back(replay_range, b, streams, bytes, first, last, &status[w]);
+            } catch (const std::system_error &) { // no thread: replay the range here
+                replay_range(b, streams, bytes, first, last, &status[w]);
+            }
         }
         for (std::thread &t : pool)
             t.join();
+        lock.lock();
         for (int st : status)
             if (st != CIH_OK) {
                 destroy_batch(b);
@@ -1295,7 +1304,7 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
         return fail(CIH_ERROR_OUT_OF_MEMORY, "out of host memory");
     } catch (const std::system_error &) {
         destroy_batch(b);
-        return fail(CIH_ERROR_DEVICE, "could not start replay threads");
+        return fail(CIH_ERROR_DEVICE, "system error while creating the batch");
     }
     *out = b;
     return CIH_OK;
