@@ -1762,6 +1762,25 @@ int finish_server_batch(cih_server *sv, int k) {
     return CIH_OK;
 }
 
+// A batch that failed to launch or to finish: its requests complete with the
+// error and give back their pool space.  Copies and kernels it already queued
+// on the lane are waited for before its buffers are freed.
+void abort_server_batch(cih_server *sv, int k, server_batch *b, int status) {
+    sv->error = status;
+    hipStreamSynchronize(sv->lanes[k].stream);
+    (void)hipGetLastError();
+    for (server_request *r : b->requests) {
+        sv->allocator.release(r->pool_at, r->bytes);
+        r->status = status;
+        r->t_done = now_ms();
+        sv->completed.push_back(r->id);
+    }
+    b->release();
+    delete b;
+    sv->inflight[k] = nullptr;
+    sv->done.notify_all();
+}
+
 void server_scheduler(cih_server *sv) {
     cih_device *d = sv->device;
     hipSetDevice(d->ordinal);
@@ -1777,18 +1796,8 @@ void server_scheduler(cih_server *sv) {
             if (q == hipErrorNotReady)
                 continue;
             int s = q == hipSuccess ? finish_server_batch(sv, k) : CIH_ERROR_DEVICE;
-            if (s != CIH_OK) {
-                sv->error = s;
-                for (server_request *r : b->requests) {
-                    r->status = s;
-                    r->t_done = now_ms();
-                    sv->completed.push_back(r->id);
-                }
-                b->release();
-                delete b;
-                sv->inflight[k] = nullptr;
-                sv->done.notify_all();
-            }
+            if (s != CIH_OK)
+                abort_server_batch(sv, k, b, s);
         }
         for (int k = 0; k < 2 && !sv->prepared.empty(); ++k) {
             if (sv->inflight[k])
@@ -1821,19 +1830,8 @@ void server_scheduler(cih_server *sv) {
                     inflight += sv->inflight[j]->requests.size();
             sv->max_inflight_requests = std::max(sv->max_inflight_requests, inflight);
             int s = launch_server_batch(sv, k, b);
-            if (s != CIH_OK) {
-                sv->error = s;
-                for (server_request *r : b->requests) {
-                    sv->allocator.release(r->pool_at, r->bytes);
-                    r->status = s;
-                    r->t_done = now_ms();
-                    sv->completed.push_back(r->id);
-                }
-                b->release();
-                delete b;
-                sv->inflight[k] = nullptr;
-                sv->done.notify_all();
-            }
+            if (s != CIH_OK)
+                abort_server_batch(sv, k, b, s);
             busy = true;
         }
         if (sv->stopping && sv->arrived.empty() && sv->prepared.empty() && !sv->inflight[0] &&
