@@ -79,7 +79,7 @@ enum counter {
     COUNTER_RUNS = 2,
     COUNTER_SHADOW = 3,
     COUNTER_OVERFLOW = 4,
-    COUNTER_QUEUE = 5,
+    COUNTER_QUEUE = 5, // coalesce work list: (entries << 40) + rows
     COUNTER_PIXEL_DRAWS = 6, // pixels composited by k_fine (main and shadow layers)
     COUNTER_COUNT = 8
 };
@@ -102,6 +102,9 @@ struct frame {
     float *shadow;
     uint64_t shadow_cap;
     unsigned long long *counters;
+    // coalesce work list: one entry per (draw, pass) with rows, in the order
+    // k_runs appended them: (first row in the list << 20) | (2 * draw + pass)
+    unsigned long long *queue;
     // fine-stage work list: (canvas, row, segment) items
     const uint32_t *items;
     uint32_t nitems;

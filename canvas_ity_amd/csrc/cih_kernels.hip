@@ -5,13 +5,15 @@
 //                  captured in HIP graphs without memset nodes)
 //   k_geometry     one workgroup per draw: flatten (one thread per cubic),
 //                  dash and stroke (one thread per subpath) -> closed polygons
-//   k_runs         one workgroup per draw and pass (main, shadow): clip
-//                  (hpp:2205-2238), signed-area runs (hpp:2109-2170) bucketed
-//                  per pixel row; the shadow pass also sizes its image
-//   k_coalesce     one wave per (draw, pass) and row stride: each row's runs
-//                  sorted, coalesced and summed along the row with the
-//                  reference's float arithmetic (hpp:2239-2252) -> coverage
-//                  steps (x, running sum) in x order
+//   k_runs         one workgroup per draw and pass (main, and shadow when the
+//                  frame has shadows): clip (hpp:2205-2238), signed-area runs
+//                  (hpp:2109-2170) bucketed per pixel row; the shadow pass
+//                  also sizes its image; each pass appends its rows to the
+//                  coalesce work list
+//   k_coalesce     a fixed grid of waves over the work list's rows: each
+//                  row's runs sorted, coalesced and summed along the row with
+//                  the reference's float arithmetic (hpp:2239-2252) ->
+//                  coverage steps (x, running sum) in x order
 //   k_shadow_*     shadow alpha raster, then the extended-box blur
 //                  (hpp:2453-2503) with one thread per row / per column
 //   k_fine         one wave per (canvas, row, 256-pixel segment): the fp32
@@ -567,15 +569,30 @@ struct edge_source {
     }
 };
 
-__global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F) {
+// Appends a (draw, pass) with `rows` row buckets to the coalesce work list.
+// One atomic hands out both the entry's index and its first row in the list,
+// so entries in index order have increasing first rows.  The pass's S.*
+// fields are written before the entry (k_coalesce reads them after the
+// kernel boundary).
+__device__ inline void enqueue_rows(const frame &F, int d, int pass, int rows) {
+    unsigned long long old =
+        atomicAdd(&F.counters[COUNTER_QUEUE], (1ull << 40) | (unsigned long long)rows);
+    uint32_t index = (uint32_t)(old >> 40);
+    unsigned long long first = old & ((1ull << 40) - 1);
+    F.queue[index] = (first << 20) | (unsigned long long)(2 * d + pass);
+}
+
+// k_runs runs `passes` workgroups per draw: the main pass, and the shadow pass
+// when the frame has shadows.
+__global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F, int passes) {
     __shared__ uint32_t lds[16];
     __shared__ uint64_t slot[1];
     __shared__ int s_ymin, s_ymax;
     __shared__ uint32_t s_rows[LDS_ROWS];
     __shared__ int s_b[6];
     __shared__ unsigned long long s_key;
-    int d = blockIdx.x >> 1;
-    int variant = blockIdx.x & 1;
+    int d = (int)blockIdx.x / passes;
+    int variant = (int)blockIdx.x % passes;
     if (d >= F.ndraws || overflowed(F))
         return;
     const draw_desc &D = F.draws[d];
@@ -774,6 +791,8 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F) {
     if (threadIdx.x != 0)
         return;
     if (!variant) {
+        if (rows > 0)
+            enqueue_rows(F, d, 0, rows);
         S.row0 = row0;
         S.rows = rows;
         S.rowoff = ro;
@@ -787,6 +806,8 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F) {
     S.srowoff = ro;
     S.sruns = rb;
     S.slast = -1; // raised by k_coalesce
+    if (rows > 0)
+        enqueue_rows(F, d, 1, rows);
     // Shadow bounds (hpp:2409-2426) over the runs that survive coalescing:
     // those with a non-zero delta, plus the first run in sort order.
     int b = D.border;
@@ -843,7 +864,6 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F) {
 
 #define STEP_SENTINEL 0x7fffffff
 #define COALESCE_CAP 1024 // runs sorted at once in LDS
-#define COALESCE_ROWS 32  // workgroups per draw and pass (rows are strided)
 
 // Sort key: x, then |delta|, then the negative delta first.
 __device__ inline unsigned long long run_key(run r) {
@@ -1152,31 +1172,66 @@ __device__ uint32_t coalesce_row(const run *in, uint32_t n, step *out, unsigned 
     return w.finish(n);
 }
 
-// One workgroup (one wave) per (draw, pass) and row stride.
+// The work list entry holding list row u: the last of the n entries whose
+// first row is <= u (entry 0 starts at row 0), by a 64-way search.
+__device__ inline uint32_t find_entry(const unsigned long long *queue, uint32_t n, uint64_t u) {
+    int lane = threadIdx.x & 63;
+    uint32_t lo = 0, hi = n; // answer in [lo, hi); queue[lo] starts at or before u
+    while (hi - lo > 1) {
+        uint32_t stride = (hi - lo + 63) / 64;
+        uint32_t p = lo + (uint32_t)lane * stride;
+        bool before = p < hi && (queue[p] >> 20) <= u;
+        int c = __popcll(__ballot(before)); // >= 1: lane 0 probes lo
+        lo = lo + (uint32_t)(c - 1) * stride;
+        hi = min(hi, lo + stride);
+    }
+    return lo;
+}
+
+// Row buckets taken COALESCE_CHUNK at a time from the work list; workgroup g
+// takes chunks g, g + G, ... so the rows of one large draw spread over many
+// workgroups.  Only rows that exist are visited: no workgroup is launched for
+// a draw's missing shadow pass or for row strides a small draw does not have.
+#define COALESCE_CHUNK 8
+#define COALESCE_GRID (256 * 19) // one wave each: 19 fit a CU's LDS
+
+// One workgroup (one wave) takes rows of the coalesce work list.
 __global__ __launch_bounds__(64) void k_coalesce(frame F) {
     __shared__ unsigned long long buf[COALESCE_CAP];
     __shared__ uint32_t s_count;
-    int d = blockIdx.x >> 1;
-    int variant = blockIdx.x & 1;
-    if (d >= F.ndraws || overflowed(F))
+    if (overflowed(F))
         return;
-    const draw_desc &D = F.draws[d];
-    draw_state &S = F.state[d];
-    if (variant && !D.shadow)
-        return;
-    int rows = variant ? S.srows : S.rows;
-    const uint32_t *off = F.words + (variant ? S.srowoff : S.rowoff);
-    uint64_t base = variant ? S.sruns : S.runs;
-    int last = -1;
-    for (int r = blockIdx.y; r < rows; r += gridDim.y) {
-        uint32_t b = off[r], e = off[r + 1];
-        if (b == e)
-            continue;
-        if (coalesce_row(F.runs + base + b, e - b, F.steps + base + b, buf, s_count))
-            last = r;
+    unsigned long long q = __atomic_load_n(&F.counters[COUNTER_QUEUE], __ATOMIC_RELAXED);
+    uint32_t nentries = (uint32_t)(q >> 40);
+    uint64_t total = q & ((1ull << 40) - 1);
+    uint64_t nchunks = (total + COALESCE_CHUNK - 1) / COALESCE_CHUNK;
+    for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        uint64_t u = chunk * COALESCE_CHUNK;
+        uint64_t end = min(u + COALESCE_CHUNK, total);
+        uint32_t i = find_entry(F.queue, nentries, u);
+        while (u < end && i < nentries) {
+            unsigned long long e = F.queue[i];
+            uint64_t first = e >> 20;
+            uint64_t next = i + 1 < nentries ? (F.queue[i + 1] >> 20) : total;
+            int dp = (int)(e & 0xfffff), d = dp >> 1, variant = dp & 1;
+            const draw_state &S = F.state[d];
+            const uint32_t *off = F.words + (variant ? S.srowoff : S.rowoff);
+            uint64_t base = variant ? S.sruns : S.runs;
+            int r0 = (int)(u - first), r1 = (int)(min(end, next) - first);
+            int last = -1;
+            for (int r = r0; r < r1; ++r) {
+                uint32_t b = off[r], x = off[r + 1];
+                if (b == x)
+                    continue;
+                if (coalesce_row(F.runs + base + b, x - b, F.steps + base + b, buf, s_count))
+                    last = r;
+            }
+            if (variant && last >= 0 && threadIdx.x == 0)
+                atomicMax(&F.state[d].slast, S.srow0 + last);
+            u = min(end, next);
+            ++i;
+        }
     }
-    if (variant && last >= 0 && threadIdx.x == 0)
-        atomicMax(&S.slast, S.srow0 + last);
 }
 
 // Reading steps.  A fine workgroup is one wave owning FINE_SEGMENT pixels
@@ -2096,10 +2151,10 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
         k_geometry<<<F.ndraws, GEO_THREADS, 0, stream>>>(F);
     MARK(2);
     if (F.ndraws > 0)
-        k_runs<<<2 * F.ndraws, GEO_THREADS, 0, stream>>>(F);
+        k_runs<<<(any_shadow ? 2 : 1) * F.ndraws, GEO_THREADS, 0, stream>>>(F, any_shadow ? 2 : 1);
     MARK(3);
     if (F.ndraws > 0)
-        k_coalesce<<<dim3(2 * F.ndraws, COALESCE_ROWS), 64, 0, stream>>>(F);
+        k_coalesce<<<COALESCE_GRID, 64, 0, stream>>>(F);
     MARK(4);
     int ry = (shadow_rows + 63) / 64, cx = (shadow_cols + 63) / 64;
     if (F.ndraws > 0 && any_shadow)

@@ -323,6 +323,11 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
     return p.bytes;
 }
 
+// A lane's state buffer for frames of up to n draws: the draw_state records,
+// then the coalesce work list (at most two entries per draw).
+static_assert(sizeof(draw_state) % 8 == 0, "the work list after the records is 8-byte aligned");
+size_t state_bytes(size_t n) { return (sizeof(draw_state) + 16) * std::max<size_t>(n, 1); }
+
 int ensure_arenas(exec_lane &L, uint64_t pts, uint64_t words, uint64_t runs, uint64_t shadow) {
     int s;
     if ((s = L.points.reserve(8 * pts)) || (s = L.words.reserve(4 * words)) ||
@@ -399,7 +404,7 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
                 const frame_shape &shape, bool check) {
     int s;
     if (check) {
-        if ((s = L.state.reserve(sizeof(draw_state) * std::max<size_t>(n, 1))) ||
+        if ((s = L.state.reserve(state_bytes(n))) ||
             (s = L.counters.reserve(64)) || (s = L.host_counters.reserve(64)))
             return s;
         uint64_t want_points = std::max<uint64_t>(1 << 22, 64 * shape.in_points);
@@ -418,6 +423,7 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
         F.payload = payload;
         F.draws = draws;
         F.state = (draw_state *)L.state.ptr;
+        F.queue = (unsigned long long *)((char *)L.state.ptr + sizeof(draw_state) * n);
         F.canvases = canvases_dev;
         F.images = (const image_desc *)d->image_table.ptr;
         F.ndraws = (int32_t)n;
@@ -1070,6 +1076,7 @@ struct cih_batch {
     dbuf frame_canvases, frame_items;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::vector<void *> graph_buffers; // the lane buffers the graph's kernels use
     bool ran = false;
     bool use_graph = true;
 };
@@ -1088,11 +1095,26 @@ void replay_range(cih_batch *b, const void *const *streams, const size_t *bytes,
     }
 }
 
-void destroy_batch(cih_batch *b) {
+// The device buffers a captured frame bakes into its kernel arguments.  Any
+// later growth of the lane (another canvas or batch on the device) moves some
+// of them, and the graph must then be captured again.
+std::vector<void *> lane_buffers(const exec_lane &L) {
+    return {L.points.ptr, L.words.ptr, L.runs.ptr, L.steps.ptr, L.shadow.ptr, L.counters.ptr,
+            L.state.ptr};
+}
+
+void drop_graph(cih_batch *b) {
     if (b->exec)
         hipGraphExecDestroy(b->exec);
     if (b->graph)
         hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_buffers.clear();
+}
+
+void destroy_batch(cih_batch *b) {
+    drop_graph(b);
     for (cih_canvas *v : b->views)
         delete v;
     dbuf *all[] = {&b->bitmaps, &b->all_canvases, &b->prefix_dev, &b->srgb, &b->payload,
@@ -1318,6 +1340,12 @@ int cih_batch_execute(cih_batch *b, int to_host) {
     int s;
     if ((s = set_device(d)))
         return s;
+    if (b->exec && b->graph_buffers != lane_buffers(d->main)) {
+        // the lane grew since the capture: run checked and capture again
+        HIP_TRY(hipStreamSynchronize(d->stream));
+        drop_graph(b);
+        b->ran = false;
+    }
     if (!b->ran) {
         // first run: checked frames size the arenas; then capture the graph
         if ((s = enqueue_batch(b, true)))
@@ -1339,7 +1367,9 @@ int cih_batch_execute(cih_batch *b, int to_host) {
             if (e != hipSuccess) {
                 (void)hipGetLastError();
                 b->use_graph = false; // fall back to stream launches of the same kernels
-                b->exec = nullptr;
+                drop_graph(b);
+            } else {
+                b->graph_buffers = lane_buffers(d->main);
             }
         }
     } else if (b->exec && !d->profiling) {
@@ -1687,7 +1717,7 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
         shadow = std::max(shadow, f.shape.shadow_bound);
         ndraws = std::max(ndraws, f.ndraws);
     }
-    if ((s = L.state.reserve(sizeof(draw_state) * std::max<size_t>(ndraws, 1))) ||
+    if ((s = L.state.reserve(state_bytes(ndraws))) ||
         (s = L.counters.reserve(64)) ||
         (s = ensure_arenas(L, std::max<uint64_t>(64 * in_points, 1 << 22),
                            std::max<uint64_t>(8 * in_points + 64 * ndraws, 1 << 22),
