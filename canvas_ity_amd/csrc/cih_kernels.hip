@@ -1188,48 +1188,158 @@ __device__ inline uint32_t find_entry(const unsigned long long *queue, uint32_t 
     return lo;
 }
 
-// Row buckets taken COALESCE_CHUNK at a time from the work list; workgroup g
-// takes chunks g, g + G, ... so the rows of one large draw spread over many
-// workgroups.  Only rows that exist are visited: no workgroup is launched for
-// a draw's missing shadow pass or for row strides a small draw does not have.
-#define COALESCE_CHUNK 8
-#define COALESCE_GRID (256 * 19) // one wave each: 19 fit a CU's LDS
+// A row of at most SMALL_ROW runs, coalesced by one lane: the keys sorted by
+// a bitonic network in registers, then the same pixel sums and running sum as
+// step_writer, in the same order.  Returns the number of steps.
+#define SMALL_ROW 32
 
-// One workgroup (one wave) takes rows of the coalesce work list.
+__device__ inline uint32_t coalesce_small(const run *in, uint32_t n, step *out) {
+    unsigned long long k[SMALL_ROW];
+#pragma unroll
+    for (int i = 0; i < SMALL_ROW; ++i)
+        k[i] = (uint32_t)i < n ? run_key(in[i]) : ~0ull;
+#pragma unroll
+    for (int size = 2; size <= SMALL_ROW; size <<= 1)
+#pragma unroll
+        for (int j = size >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < SMALL_ROW; ++i) {
+                int p = i ^ j;
+                if (p > i) {
+                    unsigned long long lo = k[i] < k[p] ? k[i] : k[p];
+                    unsigned long long hi = k[i] < k[p] ? k[p] : k[i];
+                    bool up = (i & size) == 0;
+                    k[i] = up ? lo : hi;
+                    k[p] = up ? hi : lo;
+                }
+            }
+    uint32_t count = 0;
+    int last_x = -1, open_x = -1;
+    float sum = 0.0f, open_sum = 0.0f;
+    bool open_nz = false;
+#pragma unroll
+    for (int i = 0; i < SMALL_ROW; ++i) {
+        if ((uint32_t)i < n) {
+            int x = key_x(k[i]);
+            float d = key_delta(k[i]);
+            if (x != open_x) {
+                if (open_x >= 0 && open_nz) {
+                    sum = sum + open_sum;
+                    step v = {open_x, sum};
+                    out[count++] = v;
+                    last_x = open_x;
+                }
+                open_x = x;
+                open_sum = d;
+                open_nz = d != 0.0f;
+            } else {
+                open_sum = open_sum + d;
+                open_nz = open_nz || d != 0.0f;
+            }
+        }
+    }
+    if (open_x >= 0 && open_nz) {
+        sum = sum + open_sum;
+        step v = {open_x, sum};
+        out[count++] = v;
+        last_x = open_x;
+    }
+    step pad = {STEP_SENTINEL, __int_as_float(last_x)};
+    for (uint32_t i = count; i < n; ++i)
+        out[i] = pad;
+    return count;
+}
+
+// The work list's rows, taken `per` at a time: row (u, its draw, pass, row
+// index and run bucket) for lane t < per of a wave, chunk c holding list rows
+// [c * per, c * per + per).
+struct list_row {
+    bool valid;
+    int d, pass, r;
+    uint32_t b, n;
+    uint64_t base;
+};
+
+__device__ inline list_row work_row(const frame &F, uint32_t nentries, uint64_t total,
+                                    uint64_t chunk, int per) {
+    int lane = threadIdx.x & 63;
+    uint64_t u0 = chunk * (uint64_t)per;
+    uint32_t i = find_entry(F.queue, nentries, u0);
+    uint64_t u = u0 + (uint64_t)lane;
+    list_row w;
+    w.valid = lane < per && u < total;
+    // this lane's entry: a walk forward from the chunk's first
+    while (w.valid && i + 1 < nentries && (F.queue[i + 1] >> 20) <= u)
+        ++i;
+    w.d = w.pass = w.r = 0;
+    w.b = w.n = 0;
+    w.base = 0;
+    if (w.valid) {
+        unsigned long long e = F.queue[i];
+        int dp = (int)(e & 0xfffff);
+        w.d = dp >> 1;
+        w.pass = dp & 1;
+        w.r = (int)(u - (e >> 20));
+        const draw_state &S = F.state[w.d];
+        const uint32_t *off = F.words + (w.pass ? S.srowoff : S.rowoff);
+        w.base = w.pass ? S.sruns : S.runs;
+        w.b = off[w.r];
+        w.n = off[w.r + 1] - w.b;
+    }
+    return w;
+}
+
+// Coalescing runs in two kernels over the work list, so each keeps its own
+// register budget: k_coalesce gives each row of at most SMALL_ROW runs one
+// lane (64 rows per wave); k_coalesce_wide gives each longer row the whole
+// wave (coalesce_row), taking the list WIDE_CHUNK rows at a time so long rows
+// spread over many waves.  Workgroup g takes chunks g, g + G, ...
+#define COALESCE_GRID (256 * 16)
+#define WIDE_CHUNK 8
+
 __global__ __launch_bounds__(64) void k_coalesce(frame F) {
-    __shared__ unsigned long long buf[COALESCE_CAP];
-    __shared__ uint32_t s_count;
     if (overflowed(F))
         return;
     unsigned long long q = __atomic_load_n(&F.counters[COUNTER_QUEUE], __ATOMIC_RELAXED);
     uint32_t nentries = (uint32_t)(q >> 40);
     uint64_t total = q & ((1ull << 40) - 1);
-    uint64_t nchunks = (total + COALESCE_CHUNK - 1) / COALESCE_CHUNK;
+    uint64_t nchunks = (total + 63) / 64;
     for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-        uint64_t u = chunk * COALESCE_CHUNK;
-        uint64_t end = min(u + COALESCE_CHUNK, total);
-        uint32_t i = find_entry(F.queue, nentries, u);
-        while (u < end && i < nentries) {
-            unsigned long long e = F.queue[i];
-            uint64_t first = e >> 20;
-            uint64_t next = i + 1 < nentries ? (F.queue[i + 1] >> 20) : total;
-            int dp = (int)(e & 0xfffff), d = dp >> 1, variant = dp & 1;
-            const draw_state &S = F.state[d];
-            const uint32_t *off = F.words + (variant ? S.srowoff : S.rowoff);
-            uint64_t base = variant ? S.sruns : S.runs;
-            int r0 = (int)(u - first), r1 = (int)(min(end, next) - first);
-            int last = -1;
-            for (int r = r0; r < r1; ++r) {
-                uint32_t b = off[r], x = off[r + 1];
-                if (b == x)
-                    continue;
-                if (coalesce_row(F.runs + base + b, x - b, F.steps + base + b, buf, s_count))
-                    last = r;
-            }
-            if (variant && last >= 0 && threadIdx.x == 0)
-                atomicMax(&F.state[d].slast, S.srow0 + last);
-            u = min(end, next);
-            ++i;
+        list_row w = work_row(F, nentries, total, chunk, 64);
+        if (w.valid && w.n > 0 && w.n <= SMALL_ROW) {
+            uint32_t count = coalesce_small(F.runs + w.base + w.b, w.n, F.steps + w.base + w.b);
+            if (w.pass && count)
+                atomicMax(&F.state[w.d].slast, F.state[w.d].srow0 + w.r);
+        }
+    }
+}
+
+__global__ __launch_bounds__(64) void k_coalesce_wide(frame F) {
+    __shared__ unsigned long long buf[COALESCE_CAP];
+    __shared__ uint32_t s_count;
+    if (overflowed(F))
+        return;
+    int lane = threadIdx.x & 63;
+    unsigned long long q = __atomic_load_n(&F.counters[COUNTER_QUEUE], __ATOMIC_RELAXED);
+    uint32_t nentries = (uint32_t)(q >> 40);
+    uint64_t total = q & ((1ull << 40) - 1);
+    uint64_t nchunks = (total + WIDE_CHUNK - 1) / WIDE_CHUNK;
+    for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+        list_row w = work_row(F, nentries, total, chunk, WIDE_CHUNK);
+        unsigned long long wide = __ballot(w.valid && w.n > SMALL_ROW);
+        while (wide) {
+            int l = __ffsll((long long)wide) - 1;
+            wide &= wide - 1;
+            int d = __builtin_amdgcn_readlane(w.d, l), pass = __builtin_amdgcn_readlane(w.pass, l);
+            int r = __builtin_amdgcn_readlane(w.r, l);
+            uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)w.b, l);
+            uint32_t n = (uint32_t)__builtin_amdgcn_readlane((int)w.n, l);
+            uint64_t base =
+                ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(w.base >> 32), l) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)w.base, l);
+            uint32_t count = coalesce_row(F.runs + base + b, n, F.steps + base + b, buf, s_count);
+            if (pass && count && lane == 0)
+                atomicMax(&F.state[d].slast, F.state[d].srow0 + r);
         }
     }
 }
@@ -2153,8 +2263,10 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
     if (F.ndraws > 0)
         k_runs<<<(any_shadow ? 2 : 1) * F.ndraws, GEO_THREADS, 0, stream>>>(F, any_shadow ? 2 : 1);
     MARK(3);
-    if (F.ndraws > 0)
+    if (F.ndraws > 0) {
         k_coalesce<<<COALESCE_GRID, 64, 0, stream>>>(F);
+        k_coalesce_wide<<<COALESCE_GRID, 64, 0, stream>>>(F);
+    }
     MARK(4);
     int ry = (shadow_rows + 63) / 64, cx = (shadow_cols + 63) / 64;
     if (F.ndraws > 0 && any_shadow)
