@@ -1177,11 +1177,13 @@ __device__ uint32_t coalesce_row(const run *in, uint32_t n, step *out, unsigned 
 __device__ inline uint32_t find_entry(const unsigned long long *queue, uint32_t n, uint64_t u) {
     int lane = threadIdx.x & 63;
     uint32_t lo = 0, hi = n; // answer in [lo, hi); queue[lo] starts at or before u
-    while (hi - lo > 1) {
+    for (int level = 0; level < 8 && hi - lo > 1; ++level) { // 64^8 > any list
         uint32_t stride = (hi - lo + 63) / 64;
         uint32_t p = lo + (uint32_t)lane * stride;
         bool before = p < hi && (queue[p] >> 20) <= u;
         int c = __popcll(__ballot(before)); // >= 1: lane 0 probes lo
+        if (c == 0)
+            break;
         lo = lo + (uint32_t)(c - 1) * stride;
         hi = min(hi, lo + stride);
     }
@@ -1279,12 +1281,23 @@ __device__ inline list_row work_row(const frame &F, uint32_t nentries, uint64_t 
         int dp = (int)(e & 0xfffff);
         w.d = dp >> 1;
         w.pass = dp & 1;
-        w.r = (int)(u - (e >> 20));
-        const draw_state &S = F.state[w.d];
-        const uint32_t *off = F.words + (w.pass ? S.srowoff : S.rowoff);
+        uint64_t first = e >> 20;
+        const draw_state &S = F.state[w.d < F.ndraws ? w.d : 0];
+        uint64_t rowoff = w.pass ? S.srowoff : S.rowoff;
+        int rows = w.pass ? S.srows : S.rows;
         w.base = w.pass ? S.sruns : S.runs;
-        w.b = off[w.r];
-        w.n = off[w.r + 1] - w.b;
+        // a row the list and the draw's state agree on, inside the arenas
+        w.valid = w.d < F.ndraws && u >= first && u - first < (uint64_t)rows &&
+                  rowoff + (uint64_t)rows < F.words_cap;
+        if (w.valid) {
+            w.r = (int)(u - first);
+            const uint32_t *off = F.words + rowoff;
+            w.b = off[w.r];
+            w.n = off[w.r + 1] - w.b;
+            w.valid = w.base + w.b + w.n <= F.runs_cap;
+        }
+        if (!w.valid)
+            atomicMax(&F.counters[COUNTER_OVERFLOW], 1ull); // the host re-runs and reports
     }
     return w;
 }
