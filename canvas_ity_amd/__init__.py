@@ -74,6 +74,8 @@ SIGNATURES = {
                                       _c.c_int, _c.POINTER(_P), _c.POINTER(_c.c_size_t)]),
     "cih_scene_seed": (_c.c_uint64, [_c.c_uint64, _c.c_uint64]),
     "cih_free": (None, [_P]),
+    "cih_path_contains": (_c.c_int, [_P, _c.c_size_t, _P, _c.c_size_t, _c.c_float, _c.c_float,
+                                     _c.POINTER(_c.c_int)]),
 }
 
 _lib = None
@@ -116,6 +118,19 @@ def scene(config, seed, width=0, height=0, shapes=0, flags=0):
         return _c.string_at(out, size.value)
     finally:
         lib.cih_free(out)
+
+
+def path_contains(points, subpaths, x, y):
+    """is_point_in_path for a device-space path: points (n, 2) float32 (a start
+    point and 3 per cubic per subpath), subpaths (m, 2) uint32 (count, closed).
+    Runs on the host (cih_path_contains); no device is needed."""
+    pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 2)
+    sub = np.ascontiguousarray(subpaths, dtype=np.uint32).reshape(-1, 2)
+    inside = _c.c_int()
+    _check(library().cih_path_contains(pts.ctypes.data_as(_P), pts.shape[0],
+                                       sub.ctypes.data_as(_P), sub.shape[0], float(x), float(y),
+                                       _c.byref(inside)), "cih_path_contains")
+    return bool(inside.value)
 
 
 def scene_seed(seed, index):

@@ -12,6 +12,7 @@
 
 #include "cih_abi.h"
 #include "cih_device.hpp"
+#include "cih_geometry.hpp"
 #include "cih_stream.h"
 
 #include <algorithm>
@@ -1056,6 +1057,72 @@ int cih_trace_runs(cih_canvas *c, int draw, int pass, int32_t *xs, int32_t *ys, 
 }
 
 } // extern "C"
+
+int cih_path_contains(const float *points, size_t npoints, const uint32_t *subpaths,
+                      size_t nsubpaths, float x, float y, int *inside) {
+    if (!inside || (npoints && !points) || (nsubpaths && !subpaths))
+        return fail(CIH_ERROR_INVALID, "invalid path arguments");
+    *inside = 0;
+    uint64_t total = 0;
+    for (size_t s = 0; s < nsubpaths; ++s) {
+        uint32_t c = subpaths[2 * s];
+        if (c == 0 || (c - 1) % 3 != 0)
+            return fail(CIH_ERROR_MALFORMED, "a subpath holds a start point and 3 per cubic");
+        total += c;
+    }
+    if (total != npoints)
+        return fail(CIH_ERROR_MALFORMED, "subpath counts do not add up to the points");
+    try {
+        // path_to_lines(false): start point, then each cubic's flattening
+        // with no angular test (fills emit end points only)
+        const cih::f2 *in = (const cih::f2 *)points;
+        std::vector<cih::f2> line;
+        std::vector<size_t> ends;
+        struct sink {
+            std::vector<cih::f2> *v;
+            void point(cih::f2 p) { v->push_back(p); }
+        } out = {&line};
+        size_t index = 0;
+        for (size_t s = 0; s < nsubpaths; ++s) {
+            size_t ending = index + subpaths[2 * s];
+            cih::f2 p1 = in[index++];
+            line.push_back(p1);
+            for (; index < ending; index += 3) {
+                cih::add_bezier(out, p1, in[index], in[index + 1], in[index + 2], -1.0f);
+                p1 = in[index + 2];
+            }
+            ends.push_back(line.size());
+        }
+        // crossings of the horizontal through (x, y), each subpath implicitly
+        // closed; a point on an edge counts as inside
+        int winding = 0;
+        size_t beginning = 0;
+        for (size_t s = 0; s < ends.size(); ++s) {
+            size_t ending = ends[s];
+            for (size_t i = beginning; i < ending; ++i) {
+                cih::f2 from = line[i], to = line[i + 1 < ending ? i + 1 : beginning];
+                if ((from.y < y && y <= to.y) || (to.y < y && y <= from.y)) {
+                    float side = cih::dot(cih::perp(cih::sub(to, from)),
+                                          cih::sub(cih::mk(x, y), from));
+                    if (side == 0.0f) {
+                        *inside = 1;
+                        return CIH_OK;
+                    }
+                    winding += side > 0.0f ? 1 : -1;
+                } else if (from.y == y && y == to.y &&
+                           ((from.x <= x && x <= to.x) || (to.x <= x && x <= from.x))) {
+                    *inside = 1;
+                    return CIH_OK;
+                }
+            }
+            beginning = ending;
+        }
+        *inside = winding != 0;
+    } catch (const std::bad_alloc &) {
+        return fail(CIH_ERROR_OUT_OF_MEMORY, "out of host memory");
+    }
+    return CIH_OK;
+}
 
 // ---------------------------------------------------------------------------
 // Batches: many independent canvases rendered by shared frames.

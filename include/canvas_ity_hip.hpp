@@ -18,8 +18,9 @@
 // put_image_data) or when the pending list grows large.  Results match the
 // reference's immediate mode because nothing else observes pixels.
 //
-// Not provided (out of scope here): text (set_font, fill_text, stroke_text,
-// measure_text), is_point_in_path and clip().  Extension: set_fill_rule.
+// is_point_in_path is a host-side query answered by the library without the
+// device.  Not provided (out of scope here): text (set_font, fill_text,
+// stroke_text, measure_text) and clip().  Extension: set_fill_rule.
 #ifndef CANVAS_ITY_HIP_HPP
 #define CANVAS_ITY_HIP_HPP
 
@@ -418,6 +419,21 @@ public:
     void fill() { emit_path(CIH_DRAW_FILL_PATH, fill_brush); }
     void stroke() { emit_path(CIH_DRAW_STROKE_PATH, stroke_brush); }
     void set_fill_rule(int new_rule) { rule = new_rule == evenodd ? evenodd : nonzero; }
+    // hpp:843, 3101-3132: the current path's fill flattening and crossing
+    // test, evaluated on the host by the library (no device work).
+    bool is_point_in_path(float x, float y) {
+        std::vector<uint32_t> info(2 * subpaths.size());
+        for (size_t k = 0; k < subpaths.size(); ++k) {
+            info[2 * k] = (uint32_t)subpaths[k].count;
+            info[2 * k + 1] = subpaths[k].closed ? 1u : 0u;
+        }
+        int inside = 0;
+        if (cih_path_contains(points.empty() ? 0 : &points[0].x, points.size(),
+                              info.empty() ? 0 : &info[0], subpaths.size(), x, y,
+                              &inside) != 0)
+            return false;
+        return inside != 0;
+    }
 
     void clear_rectangle(float x, float y, float width, float height) {
         composite_operation saved_operation = global_composite_operation;

@@ -189,6 +189,15 @@ CIH_API int cih_trace_runs(cih_canvas *canvas, int draw, int pass, int32_t *xs, 
  * canvas API into this canvas.  The stream header's size must match. */
 CIH_API int cih_canvas_replay(cih_canvas *canvas, const void *stream, size_t bytes);
 
+/* is_point_in_path (canvas_ity.hpp:3101-3132): whether (x, y) lies inside
+ * the path by the non-zero rule or on its boundary, for a path in device
+ * space given as the packets carry it (a start point and 3 points per cubic
+ * per subpath; subpaths as (count, closed) pairs).  The path is flattened for
+ * filling (hpp:1495-1524) and crossings are counted with the reference's
+ * float arithmetic.  A host-side query: it needs no device. */
+CIH_API int cih_path_contains(const float *points, size_t npoints, const uint32_t *subpaths,
+                              size_t nsubpaths, float x, float y, int *inside);
+
 /* ---- batches: many independent canvases in shared frames ------------------
  * A batch replays every command stream through canvas_ity_hip.hpp (host
  * threads in parallel), keeps all canvases, draw records and packets resident

@@ -61,6 +61,12 @@ def _load(kind):
         fn = lib.cih_ref_lines_to_runs
         fn.restype = ctypes.c_long
         fn.argtypes = [_P, ctypes.c_float, ctypes.c_float, ctypes.c_int, _P, ctypes.c_long]
+        fn = lib.cih_ref_path
+        fn.restype = ctypes.c_long
+        fn.argtypes = [_P, _P, ctypes.c_long, _P, ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+        fn = lib.cih_ref_is_point_in_path
+        fn.restype = ctypes.c_int
+        fn.argtypes = [_P, ctypes.c_float, ctypes.c_float]
     fns["lib"] = lib
     _libs[kind] = fns
     return fns

@@ -129,4 +129,31 @@ long cih_ref_lines_to_runs(void *handle, float offset_x, float offset_y, int pad
     return n;
 }
 
+// The current path in device space as the reference stores it (hpp:1162):
+// up to `capacity` points and `subpath_capacity` (count, closed) pairs;
+// returns the number of points, *subpaths the number of subpaths.
+long cih_ref_path(void *handle, float *points, long capacity, int *subpath_info,
+                  long subpath_capacity, long *subpaths)
+{
+    canvas *c = static_cast<canvas *>(handle);
+    long n = (long)c->path.points.size();
+    for (long k = 0; k < n && k < capacity; ++k) {
+        points[2 * k + 0] = c->path.points[(size_t)k].x;
+        points[2 * k + 1] = c->path.points[(size_t)k].y;
+    }
+    long s = (long)c->path.subpaths.size();
+    for (long k = 0; k < s && k < subpath_capacity; ++k) {
+        subpath_info[2 * k + 0] = (int)c->path.subpaths[(size_t)k].count;
+        subpath_info[2 * k + 1] = c->path.subpaths[(size_t)k].closed ? 1 : 0;
+    }
+    *subpaths = s;
+    return n;
+}
+
+// The reference's is_point_in_path (hpp:3101-3132) on the current path.
+int cih_ref_is_point_in_path(void *handle, float x, float y)
+{
+    return static_cast<canvas *>(handle)->is_point_in_path(x, y) ? 1 : 0;
+}
+
 } // extern "C"
