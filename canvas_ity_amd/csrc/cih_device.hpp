@@ -21,9 +21,10 @@ struct draw_desc {
     float weight_1, weight_2;
     int32_t brush;
     float start_x, start_y, end_x, end_y, start_radius, end_radius;
-    int32_t image, image_width, image_height, repetition;
+    int32_t image, image_width, image_height, repetition; // clip: image = the new mask id
     uint32_t points_at, npoints, subpaths_at, nsubpaths;
     uint32_t dash_at, ndash, colors_at, ncolors, stops_at, nstops;
+    int32_t clip_mask, pad_; // mask the draw is composited under (0: whole canvas)
 };
 
 // Results the kernels leave for the later stages of the same frame.
@@ -43,6 +44,8 @@ struct draw_state {
     int32_t srow0, srows;
     uint64_t srowoff, sruns;
     int32_t slast; // last row with a shadow step (its last pixel is never painted)
+    int32_t clast; // clip draws: last row with a coalesced run (-1: none)
+    int32_t pad_;
     // shadow image: [left, left + width) x [top, top + height) of the
     // extended canvas, stored row-major at `image` (and `image2` ping-pong)
     int32_t left, top, width, height;
@@ -58,6 +61,19 @@ struct canvas_desc {
     uint64_t bitmap; // device address of float4 pixels
     int32_t width, height;
     int32_t first_draw, ndraws;
+    uint64_t masks; // device address of the canvas' mask_desc table (0: none)
+};
+
+// A clip mask (the reference's `mask` run list, hpp:1169): per canvas row,
+// up to width + 2 pieces (x, delta) in x order, and the running visibility
+// sum after each piece as steps (x, sum) for lookups.  Row y's entries start
+// at y * (width + 2); counts[y] of them are used.
+struct mask_desc {
+    uint64_t pieces; // device address of {int32 x, float delta}
+    uint64_t steps;  // device address of {int32 x, float sum}
+    uint64_t counts; // device address of uint32 per row
+    int32_t last_row; // last row with a piece (-1: the mask is empty)
+    int32_t pad_;
 };
 
 struct run {

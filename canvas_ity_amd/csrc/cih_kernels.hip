@@ -14,13 +14,17 @@
 //                  row's runs sorted, coalesced and summed along the row with
 //                  the reference's float arithmetic (hpp:2239-2252) ->
 //                  coverage steps (x, running sum) in x order
+//   k_clip         clip draws: the path's coverage merged with the parent
+//                  mask into a new mask, one thread per canvas row
+//                  (hpp:3057-3099)
 //   k_shadow_*     shadow alpha raster, then the extended-box blur
 //                  (hpp:2453-2503) with one thread per row / per column
 //   k_fine         one wave per (canvas, row, 256-pixel segment): the fp32
 //                  pixels stay in registers while every draw of the canvas is
 //                  applied in painter's order -- coverage from the row's steps
 //                  (a 64-way search, then LDS for steps inside the segment),
-//                  paint (hpp:2265-2377) and composite (hpp:2576-2592)
+//                  paint (hpp:2265-2377) and composite (hpp:2576-2592);
+//                  k_fine<true> also applies clip-mask visibility
 //   k_readback     get_image_data's unpremultiply/sRGB/dither (hpp:3363-3380)
 // None of this is a dense contraction: the pipeline is VALU/LDS and HBM work.
 #include <hip/hip_runtime.h>
@@ -459,7 +463,8 @@ __global__ __launch_bounds__(GEO_THREADS) void k_geometry(frame F) {
     line_path lp;
     bool stroke = D.kind == CIH_DRAW_STROKE_PATH || D.kind == CIH_DRAW_STROKE_LINES;
     draw_state &S = F.state[d];
-    if (D.kind == CIH_DRAW_FILL_PATH || D.kind == CIH_DRAW_STROKE_PATH) {
+    if (D.kind == CIH_DRAW_FILL_PATH || D.kind == CIH_DRAW_STROKE_PATH ||
+        D.kind == CIH_DRAW_CLIP) {
         lp = flatten(F, D, stroke, lds, slot);
         if (!lp.pts && D.nsubpaths)
             return;
@@ -791,6 +796,7 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F, int passes) {
     if (threadIdx.x != 0)
         return;
     if (!variant) {
+        S.clast = -1; // raised by k_coalesce for clip draws
         if (rows > 0)
             enqueue_rows(F, d, 0, rows);
         S.row0 = row0;
@@ -948,9 +954,11 @@ struct step_writer {
     float sum; // the row's running sum
     int bx;
     float bs;
+    bool deltas; // write each pixel's summed delta instead of the running sum
 
-    __device__ void init(step *o) {
+    __device__ void init(step *o, bool d) {
         out = o;
+        deltas = d;
         count = 0;
         last_x = -1;
         open_x = -1;
@@ -976,7 +984,7 @@ struct step_writer {
     __device__ void close_open() {
         if (open_x >= 0 && open_nz) {
             sum = sum + open_sum;
-            emit(open_x, sum);
+            emit(open_x, deltas ? open_sum : sum);
         }
         open_x = -1;
     }
@@ -1032,7 +1040,7 @@ struct step_writer {
             float ml = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m), l));
             int xl = __builtin_amdgcn_readlane(x, l);
             sum = sum + ml;
-            emit(xl, sum);
+            emit(xl, deltas ? ml : sum);
         }
     }
     // Closes the row: flushes buffered steps and pads to n with sentinels.
@@ -1053,10 +1061,10 @@ struct step_writer {
 
 // Coalesces one row bucket in[0, n) into out[0, n); returns the step count.
 __device__ uint32_t coalesce_row(const run *in, uint32_t n, step *out, unsigned long long *buf,
-                                 uint32_t &s_count) {
+                                 uint32_t &s_count, bool deltas) {
     int lane = threadIdx.x & 63;
     step_writer w;
-    w.init(out);
+    w.init(out, deltas);
     if (n <= 64) {
         unsigned long long key = (uint32_t)lane < n ? run_key(in[lane]) : ~0ull;
         w.chunk(wave_sort(key), (int)n, false);
@@ -1195,7 +1203,7 @@ __device__ inline uint32_t find_entry(const unsigned long long *queue, uint32_t 
 // step_writer, in the same order.  Returns the number of steps.
 #define SMALL_ROW 32
 
-__device__ inline uint32_t coalesce_small(const run *in, uint32_t n, step *out) {
+__device__ inline uint32_t coalesce_small(const run *in, uint32_t n, step *out, bool deltas) {
     unsigned long long k[SMALL_ROW];
 #pragma unroll
     for (int i = 0; i < SMALL_ROW; ++i)
@@ -1227,7 +1235,7 @@ __device__ inline uint32_t coalesce_small(const run *in, uint32_t n, step *out) 
             if (x != open_x) {
                 if (open_x >= 0 && open_nz) {
                     sum = sum + open_sum;
-                    step v = {open_x, sum};
+                    step v = {open_x, deltas ? open_sum : sum};
                     out[count++] = v;
                     last_x = open_x;
                 }
@@ -1242,7 +1250,7 @@ __device__ inline uint32_t coalesce_small(const run *in, uint32_t n, step *out) 
     }
     if (open_x >= 0 && open_nz) {
         sum = sum + open_sum;
-        step v = {open_x, sum};
+        step v = {open_x, deltas ? open_sum : sum};
         out[count++] = v;
         last_x = open_x;
     }
@@ -1320,9 +1328,13 @@ __global__ __launch_bounds__(64) void k_coalesce(frame F) {
     for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
         list_row w = work_row(F, nentries, total, chunk, 64);
         if (w.valid && w.n > 0 && w.n <= SMALL_ROW) {
-            uint32_t count = coalesce_small(F.runs + w.base + w.b, w.n, F.steps + w.base + w.b);
+            bool clip = F.draws[w.d].kind == CIH_DRAW_CLIP;
+            uint32_t count =
+                coalesce_small(F.runs + w.base + w.b, w.n, F.steps + w.base + w.b, clip);
             if (w.pass && count)
                 atomicMax(&F.state[w.d].slast, F.state[w.d].srow0 + w.r);
+            if (clip && count)
+                atomicMax(&F.state[w.d].clast, F.state[w.d].row0 + w.r);
         }
     }
 }
@@ -1350,11 +1362,131 @@ __global__ __launch_bounds__(64) void k_coalesce_wide(frame F) {
             uint64_t base =
                 ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(w.base >> 32), l) << 32) |
                 (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)w.base, l);
-            uint32_t count = coalesce_row(F.runs + base + b, n, F.steps + base + b, buf, s_count);
+            bool clip = F.draws[d].kind == CIH_DRAW_CLIP;
+            uint32_t count =
+                coalesce_row(F.runs + base + b, n, F.steps + base + b, buf, s_count, clip);
             if (pass && count && lane == 0)
                 atomicMax(&F.state[d].slast, F.state[d].srow0 + r);
+            if (clip && count && lane == 0)
+                atomicMax(&F.state[d].clast, F.state[d].row0 + r);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Clip masks (hpp:3057-3099).  The reference merges the clip path's coalesced
+// runs with the current mask's runs in (y, x, |delta|) order (a tie takes the
+// mask's run first), keeps a running sum of each list per row, and appends a
+// piece wherever visibility = min(|sum_1|, 1) * min(|sum_2|, 1) changes,
+// adding to the previous piece when it has the same pixel.  It stops when
+// either list runs out, so rows after the last row of either are empty and
+// that last row ends at the exhausted list's last run.  Rows are independent,
+// so k_clip gives each row of the canvas one thread; the clip draw's steps
+// hold each pixel's summed delta (k_coalesce's delta mode).  The constructor's
+// mask (hpp:2636-2643) is mask 0: pieces (0, +1) and (width, -1) on every row.
+
+__global__ __launch_bounds__(64) void k_clip(frame F, int d) {
+    if (overflowed(F))
+        return;
+    const draw_desc &D = F.draws[d];
+    const draw_state &S = F.state[d];
+    const canvas_desc &C = F.canvases[D.canvas];
+    const mask_desc *T = (const mask_desc *)(uintptr_t)C.masks;
+    int y = (int)(blockIdx.x * 64 + threadIdx.x);
+    if (!T || y >= C.height)
+        return;
+    uint32_t cap = (uint32_t)C.width + 2;
+    const mask_desc &M = T[D.image];
+    run *out = (run *)(uintptr_t)M.pieces + (size_t)y * cap;
+    step *vis = (step *)(uintptr_t)M.steps + (size_t)y * cap;
+    int parent = D.clip_mask;
+    int last1 = S.clast;
+    int last2 = parent ? T[parent].last_row : C.height - 1;
+    uint32_t n = 0;
+    if (last1 >= 0 && last2 >= 0 && y <= min(last1, last2)) {
+        // the clip path's entries (x, summed delta) in this row
+        const step *pe = nullptr;
+        uint32_t n1 = 0;
+        if (S.rows > 0 && y >= S.row0 && y < S.row0 + S.rows) {
+            const uint32_t *ro = F.words + S.rowoff + (y - S.row0);
+            pe = F.steps + S.runs + ro[0];
+            uint32_t slots = ro[1] - ro[0];
+            while (n1 < slots && pe[n1].x != STEP_SENTINEL)
+                ++n1;
+        }
+        // the parent mask's pieces in this row
+        run whole[2];
+        whole[0].x = 0, whole[0].delta = 1.0f;
+        whole[1].x = C.width, whole[1].delta = -1.0f;
+        const run *pp = whole;
+        uint32_t n2 = 2;
+        if (parent) {
+            pp = (const run *)(uintptr_t)T[parent].pieces + (size_t)y * cap;
+            n2 = ((const uint32_t *)(uintptr_t)T[parent].counts)[y];
+        }
+        bool stop1 = y == last1, stop2 = y == last2;
+        float last = 0.0f, sum_1 = 0.0f, sum_2 = 0.0f;
+        uint32_t i1 = 0, i2 = 0;
+        for (;;) {
+            bool has1 = i1 < n1, has2 = i2 < n2;
+            if ((!has1 && !has2) || (!has1 && stop1) || (!has2 && stop2))
+                break;
+            bool first = has1 && (!has2 || pe[i1].x < pp[i2].x ||
+                                  (pe[i1].x == pp[i2].x && fabsf(pe[i1].sum) < fabsf(pp[i2].delta)));
+            int x;
+            if (first) {
+                x = pe[i1].x;
+                sum_1 = sum_1 + pe[i1].sum;
+                ++i1;
+            } else {
+                x = pp[i2].x;
+                sum_2 = sum_2 + pp[i2].delta;
+                ++i2;
+            }
+            float visibility = smin(fabsf(sum_1), 1.0f) * smin(fabsf(sum_2), 1.0f);
+            if (visibility == last)
+                continue;
+            if (n && out[n - 1].x == x) {
+                out[n - 1].delta = out[n - 1].delta + (visibility - last);
+            } else {
+                if (n == cap) { // cannot happen: pieces have distinct x in [0, width + 1]
+                    atomicMax(&F.counters[COUNTER_OVERFLOW], 1ull);
+                    break;
+                }
+                out[n].x = x;
+                out[n].delta = visibility - last;
+                ++n;
+            }
+            last = visibility;
+        }
+    }
+    // the running visibility sum render_main keeps (hpp:2603)
+    float sum = 0.0f;
+    for (uint32_t i = 0; i < n; ++i) {
+        sum = sum + out[i].delta;
+        vis[i].x = out[i].x;
+        vis[i].sum = sum;
+    }
+    ((uint32_t *)(uintptr_t)M.counts)[y] = n;
+}
+
+// The new mask's last row with a piece, for the next clip's merge.
+__global__ __launch_bounds__(64) void k_clip_finish(frame F, int d) {
+    if (overflowed(F))
+        return;
+    const draw_desc &D = F.draws[d];
+    const canvas_desc &C = F.canvases[D.canvas];
+    mask_desc *T = (mask_desc *)(uintptr_t)C.masks;
+    if (!T)
+        return;
+    const uint32_t *counts = (const uint32_t *)(uintptr_t)T[D.image].counts;
+    int last = -1;
+    for (int y = threadIdx.x; y < C.height; y += 64)
+        if (counts[y])
+            last = y;
+    last = wave_max(last);
+    if (threadIdx.x == 0)
+        T[D.image].last_row = last;
 }
 
 // Reading steps.  A fine workgroup is one wave owning FINE_SEGMENT pixels
@@ -1933,31 +2065,38 @@ __device__ __forceinline__ float4 gradient_pixel(const gradient_setup &G, const 
 // everything).
 enum { FILL_SOLID, FILL_EMPTY, FILL_GRADIENT };
 
-template <int KIND>
+template <int KIND, bool MASKED>
 __device__ __forceinline__ void composite_pixels(float4 solid, const gradient_setup &G,
                                                  const float *stops, const float *colors,
                                                  float4 (&back)[FINE_PX],
                                                  const int (&xs)[FINE_PX], int width, float py,
                                                  const float (&sums)[FINE_PX], bool evenodd,
                                                  bool clear, float galpha, int op,
+                                                 const float (&vis)[FINE_PX],
                                                  unsigned &composites) {
 #pragma clang loop unroll(full)
     for (int k = 0; k < FINE_PX; ++k) {
         float cov = coverage(sums[k], evenodd);
         bool on = xs[k] < width && (cov >= kThreshold || clear);
+        if (MASKED)
+            on = on && vis[k] >= kThreshold;
         float4 f = solid;
         if (KIND == FILL_EMPTY)
             f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (KIND == FILL_GRADIENT)
             f = gradient_pixel(G, stops, colors, mk((float)xs[k] + 0.5f, py));
         float4 b = back[k];
-        composite(b, scaled(f, cov * galpha), op, 1.0f);
+        composite(b, scaled(f, cov * galpha), op, MASKED ? vis[k] : 1.0f);
         float4 a = back[k];
         back[k] = make_float4(on ? b.x : a.x, on ? b.y : a.y, on ? b.z : a.z, on ? b.w : a.w);
         composites += on ? 1u : 0u;
     }
 }
 
+// MASKED: the frame has draws under a clip mask (their visibility is looked
+// up per pixel like coverage, hpp:2565-2604); the plain instantiation keeps
+// the registers of the common case.
+template <bool MASKED>
 __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
     __shared__ seg_lds L;
     __shared__ paint_lds P;
@@ -1991,13 +2130,14 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
         const step *st = nullptr;
         uint32_t count = 0, first = 0;
         float before = 0.0f;
-        int op = 0, flags = 0, brush = 0;
+        int op = 0, flags = 0, brush = 0, cmask = 0;
         float alpha = 0.0f;
         float4 color = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         if (dl < dend) {
             const draw_desc &D = F.draws[dl];
             const draw_state &S = F.state[dl];
             op = D.op;
+            cmask = D.clip_mask;
             flags = D.flags;
             brush = D.brush;
             alpha = D.global_alpha;
@@ -2033,6 +2173,8 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
             // row); operators without bit 8 repaint every pixel (hpp:2574).
             need_main = !(op & 8) || has_steps ||
                         coverage(before, (flags & CIH_DRAW_FLAG_EVENODD) != 0) >= kThreshold;
+            if (D.kind == CIH_DRAW_CLIP) // makes a mask, paints nothing
+                need_main = need_shadow = false;
             if (D.brush == CIH_BRUSH_COLOR && D.ncolors)
                 color = load4((const float *)(F.payload + D.colors_at));
         }
@@ -2061,6 +2203,31 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
             next = fetch(todo);
             int d = d0 + j;
             int dop = lane_int(op, j);
+            // visibility under the draw's clip mask: min(|running sum of the
+            // mask's pieces at or left of the pixel|, 1)
+            float vis[FINE_PX];
+            for (int k = 0; k < FINE_PX; ++k)
+                vis[k] = 1.0f;
+            if (MASKED) {
+                int m = lane_int(cmask, j);
+                if (m) {
+                    const mask_desc &M = ((const mask_desc *)(uintptr_t)C.masks)[m];
+                    uint32_t mn = ((const uint32_t *)(uintptr_t)M.counts)[y];
+                    float vb = 0.0f, vsums[FINE_PX];
+                    bool vin = false;
+                    if (mn)
+                        vin = segment_sums((const step *)(uintptr_t)M.steps +
+                                               (size_t)y * (size_t)(C.width + 2),
+                                           mn, x0, L, vb, vsums);
+                    bool any = false;
+                    for (int k = 0; k < FINE_PX; ++k) {
+                        vis[k] = smin(fabsf(vin ? vsums[k] : vb), 1.0f);
+                        any = any || vis[k] >= kThreshold;
+                    }
+                    if (!__ballot(any))
+                        continue; // nothing of this draw is visible in the segment
+                }
+            }
             if (lane_int((int)need_shadow, j)) {
                 const draw_desc &D = F.draws[d];
                 const draw_state &S = F.state[d];
@@ -2074,7 +2241,10 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
                     if (x < lo || x >= hi)
                         continue;
                     float sv = row[x + D.border - S.left];
-                    composite(back[k], scaled(scolor, D.global_alpha * sv), dop, 1.0f);
+                    if (MASKED && !(vis[k] >= kThreshold))
+                        continue;
+                    composite(back[k], scaled(scolor, D.global_alpha * sv), dop,
+                              MASKED ? vis[k] : 1.0f);
                     ++composites;
                 }
             }
@@ -2104,8 +2274,9 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
                 float4 p = make_float4(lane_float(color.x, j), lane_float(color.y, j),
                                        lane_float(color.z, j), lane_float(color.w, j));
                 gradient_setup none;
-                composite_pixels<FILL_SOLID>(p, none, nullptr, nullptr, back, xs, C.width, 0.0f,
-                                             sums, evenodd, clear, galpha, dop, composites);
+                composite_pixels<FILL_SOLID, MASKED>(p, none, nullptr, nullptr, back, xs,
+                                                     C.width, 0.0f, sums, evenodd, clear,
+                                                     galpha, dop, vis, composites);
                 continue;
             }
             // Gradients and patterns: the draw's descriptor (and a gradient's
@@ -2132,25 +2303,27 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
                 gradient_setup G = gradient_prepare(D);
                 float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 if (G.empty)
-                    composite_pixels<FILL_EMPTY>(zero4, G, nullptr, nullptr, back, xs, C.width, py,
-                                                 sums, evenodd, clear, galpha, dop, composites);
+                    composite_pixels<FILL_EMPTY, MASKED>(zero4, G, nullptr, nullptr, back, xs,
+                                                         C.width, py, sums, evenodd, clear,
+                                                         galpha, dop, vis, composites);
                 else if (staged)
-                    composite_pixels<FILL_GRADIENT>(zero4, G, P.stops, P.colors, back, xs,
-                                                    C.width, py, sums, evenodd, clear, galpha,
-                                                    dop, composites);
+                    composite_pixels<FILL_GRADIENT, MASKED>(zero4, G, P.stops, P.colors, back,
+                                                            xs, C.width, py, sums, evenodd,
+                                                            clear, galpha, dop, vis, composites);
                 else
-                    composite_pixels<FILL_GRADIENT>(zero4, G, gstops, gcolors, back, xs, C.width,
-                                                    py, sums, evenodd, clear, galpha, dop,
-                                                    composites);
+                    composite_pixels<FILL_GRADIENT, MASKED>(zero4, G, gstops, gcolors, back, xs,
+                                                            C.width, py, sums, evenodd, clear,
+                                                            galpha, dop, vis, composites);
                 continue;
             }
             // patterns: bicubic taps, only for the pixels that are painted
 #pragma clang loop unroll(full)
             for (int k = 0; k < FINE_PX; ++k) {
                 float cov = coverage(sums[k], evenodd);
-                if (xs[k] < C.width && (cov >= kThreshold || clear)) {
+                if (xs[k] < C.width && (cov >= kThreshold || clear) &&
+                    (!MASKED || vis[k] >= kThreshold)) {
                     float4 p = paint(F, D, mk((float)xs[k] + 0.5f, py));
-                    composite(back[k], scaled(p, cov * galpha), dop, 1.0f);
+                    composite(back[k], scaled(p, cov * galpha), dop, MASKED ? vis[k] : 1.0f);
                     ++composites;
                 }
             }
@@ -2261,7 +2434,8 @@ extern const char *const kFrameKernelNames[kFrameKernels] = {
 // of them), event k is recorded before kernel k and the last after the last
 // kernel, so the time of kernel k is elapsed(events[k], events[k + 1]).
 hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool any_shadow,
-                        hipEvent_t *events, hipStream_t stream) {
+                        const int32_t *clips, int nclips, bool masked, hipEvent_t *events,
+                        hipStream_t stream) {
 #define MARK(k)                                                                                \
     do {                                                                                       \
         if (events)                                                                            \
@@ -2280,6 +2454,11 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
         k_coalesce<<<COALESCE_GRID, 64, 0, stream>>>(F);
         k_coalesce_wide<<<COALESCE_GRID, 64, 0, stream>>>(F);
     }
+    // clip masks in draw order (clips[2k] = draw, clips[2k + 1] = canvas height)
+    for (int k = 0; k < nclips; ++k) {
+        k_clip<<<(clips[2 * k + 1] + 63) / 64, 64, 0, stream>>>(F, clips[2 * k]);
+        k_clip_finish<<<1, 64, 0, stream>>>(F, clips[2 * k]);
+    }
     MARK(4);
     int ry = (shadow_rows + 63) / 64, cx = (shadow_cols + 63) / 64;
     if (F.ndraws > 0 && any_shadow)
@@ -2294,8 +2473,13 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
     if (F.ndraws > 0 && any_shadow)
         k_blur_cols<<<dim3(F.ndraws, cx > 0 ? cx : 1), 64, 0, stream>>>(F);
     MARK(7);
-    if (F.nitems)
-        k_fine<<<FINE_XCDS * ((F.nitems + FINE_XCDS - 1) / FINE_XCDS), FINE_THREADS, 0, stream>>>(F);
+    if (F.nitems) {
+        unsigned grid = FINE_XCDS * ((F.nitems + FINE_XCDS - 1) / FINE_XCDS);
+        if (masked)
+            k_fine<true><<<grid, FINE_THREADS, 0, stream>>>(F);
+        else
+            k_fine<false><<<grid, FINE_THREADS, 0, stream>>>(F);
+    }
     MARK(8);
 #undef MARK
     return hipGetLastError();

@@ -48,8 +48,15 @@ const uint64_t kBatchPixelLimit = (uint64_t)64 << 20;
 
 // Distinct images one device holds (the image table is allocated once).
 const size_t kMaxImages = (size_t)1 << 16;
+
+// Clip mask ids one canvas may use at once (its mask table is allocated once;
+// the header reuses ids, so this bounds the depth of nested clips), and the
+// device memory its masks may take.
+const uint32_t kMaxMasks = 256;
+const uint64_t kMaskBytesLimit = (uint64_t)8 << 30;
 extern const char *const kFrameKernelNames[kFrameKernels];
 hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool any_shadow,
+                        const int32_t *clips, int nclips, bool masked,
                         hipEvent_t *events, hipStream_t stream);
 hipError_t launch_clear(void *pixels, size_t n, hipStream_t stream);
 hipError_t launch_prepare_image(const void *rgba8, void *texels, size_t n, const float *lut,
@@ -223,6 +230,12 @@ struct cih_canvas {
     dbuf bitmap;
     std::vector<unsigned char> pending;
     program program;
+    // clip masks: a table of kMaxMasks mask_desc in HBM (entry 0 unused: the
+    // whole canvas) and each id's storage, allocated when a clip defines it
+    dbuf mask_table;
+    std::vector<dbuf> mask_store;
+    std::vector<bool> mask_defined;
+    uint64_t mask_bytes = 0;
 };
 
 namespace {
@@ -243,7 +256,10 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
                   4 * (size_t)p.dashes + 16 * (size_t)p.colors + 4 * (size_t)p.stops;
     if (p.bytes < body || p.bytes > end - at || (p.bytes & 15))
         return 0;
-    if (p.kind < CIH_DRAW_FILL_PATH || p.kind > CIH_DRAW_STROKE_LINES)
+    if (p.kind < CIH_DRAW_FILL_PATH || p.kind > CIH_DRAW_CLIP)
+        return 0;
+    if (p.clip_mask >= kMaxMasks ||
+        (p.kind == CIH_DRAW_CLIP && (p.image < 1 || p.image >= (int32_t)kMaxMasks)))
         return 0;
     // subpath counts must add up to the points; bezier subpaths are 1 + 3k
     const unsigned char *sub = base + at + sizeof(p) + 8 * (size_t)p.points;
@@ -260,7 +276,10 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
         return 0;
     if ((p.brush == CIH_BRUSH_LINEAR || p.brush == CIH_BRUSH_RADIAL) && p.stops != p.colors)
         return 0;
-    if (p.brush == CIH_BRUSH_PATTERN || p.brush == CIH_BRUSH_IMAGE) {
+    if (p.kind == CIH_DRAW_CLIP) {
+        if (p.brush != CIH_BRUSH_COLOR)
+            return 0;
+    } else if (p.brush == CIH_BRUSH_PATTERN || p.brush == CIH_BRUSH_IMAGE) {
         if (p.image < 0 || (size_t)p.image >= dev->images.size() ||
             dev->images[(size_t)p.image]->width != p.image_width ||
             dev->images[(size_t)p.image]->height != p.image_height)
@@ -283,7 +302,8 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
     std::memcpy(out.shadow_color, p.shadow_color, sizeof(out.shadow_color));
     out.shadow_offset_x = p.shadow_offset_x;
     out.shadow_offset_y = p.shadow_offset_y;
-    out.shadow = p.shadow_color[3] != 0.0f ? 1 : 0;
+    out.shadow = p.shadow_color[3] != 0.0f && p.kind != CIH_DRAW_CLIP ? 1 : 0;
+    out.clip_mask = (int32_t)p.clip_mask;
     // Extended-box blur constants (hpp:2402-2405, 2453-2459), in float as
     // the reference computes them.
     float sigma_squared = 0.25f * p.shadow_blur * p.shadow_blur;
@@ -367,6 +387,8 @@ struct frame_shape {
     uint64_t shadow_bound = 0, in_points = 0;
     size_t ndraws = 0;
     std::vector<uint32_t> items; // per-canvas prefix of (row, segment) items
+    std::vector<int32_t> clips;  // (draw, canvas height) of each clip draw, in order
+    bool masked = false;         // some draw is composited under a clip mask
 };
 
 frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t ncanvases) {
@@ -381,6 +403,11 @@ frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t 
             const draw_desc &dd = host[k];
             f.in_points += dd.npoints;
             f.ndraws++;
+            if (dd.kind == CIH_DRAW_CLIP) {
+                f.clips.push_back(k);
+                f.clips.push_back(cd.height);
+            }
+            f.masked = f.masked || (dd.clip_mask != 0 && dd.kind != CIH_DRAW_CLIP);
             if (dd.shadow) {
                 f.any_shadow = true;
                 int b = dd.border;
@@ -447,7 +474,9 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
             if ((s = profile_events(d, &events)))
                 return s;
         }
-        HIP_TRY(launch_frame(F, shape.max_rows, shape.max_cols, shape.any_shadow, events,
+        HIP_TRY(launch_frame(F, shape.max_rows, shape.max_cols, shape.any_shadow,
+                             shape.clips.data(), (int)(shape.clips.size() / 2), shape.masked,
+                             events,
                              L.stream));
         d->frames++;
         if (!check)
@@ -488,6 +517,43 @@ int launch(cih_device *d, const frame_input &in, bool check) {
                        in.canvases.size(), (const uint32_t *)d->main.items.ptr, shape, check);
 }
 
+// Device storage for clip mask `id` of a canvas (kept once allocated: the id
+// is reused by later clips, which overwrite the mask when they run).
+int ensure_mask(cih_canvas *cv, uint32_t id) {
+    int s;
+    if (!cv->mask_table.ptr) {
+        if ((s = cv->mask_table.reserve(sizeof(mask_desc) * kMaxMasks)))
+            return s;
+        std::vector<mask_desc> zero(kMaxMasks);
+        std::memset(zero.data(), 0, sizeof(mask_desc) * kMaxMasks);
+        HIP_TRY(hipMemcpy(cv->mask_table.ptr, zero.data(), sizeof(mask_desc) * kMaxMasks,
+                          hipMemcpyHostToDevice));
+        cv->mask_store.resize(kMaxMasks);
+        cv->mask_defined.assign(kMaxMasks, false);
+    }
+    if (cv->mask_store[id].ptr)
+        return CIH_OK;
+    uint64_t cap = (uint64_t)cv->width + 2, rows = (uint64_t)cv->height;
+    uint64_t bytes = 16 * rows * cap + 4 * rows;
+    if (cv->mask_bytes + bytes > kMaskBytesLimit)
+        return fail(CIH_ERROR_BUDGET, "the canvas' clip masks exceed " +
+                                          std::to_string(kMaskBytesLimit) + " bytes");
+    if ((s = cv->mask_store[id].reserve(bytes)))
+        return s;
+    cv->mask_bytes += bytes;
+    char *base = (char *)cv->mask_store[id].ptr;
+    mask_desc m;
+    std::memset(&m, 0, sizeof(m));
+    m.pieces = (uint64_t)(uintptr_t)base;
+    m.steps = (uint64_t)(uintptr_t)(base + 8 * rows * cap);
+    m.counts = (uint64_t)(uintptr_t)(base + 16 * rows * cap);
+    m.last_row = -1;
+    // only the new entry: the device has written the others' last rows
+    HIP_TRY(hipMemcpy((mask_desc *)cv->mask_table.ptr + id, &m, sizeof(m),
+                      hipMemcpyHostToDevice));
+    return CIH_OK;
+}
+
 // Decodes pending packets into the canvas' resident program (payload and
 // draw records in HBM) split into frames of bounded size.
 int prepare_program(cih_canvas *cv) {
@@ -506,6 +572,15 @@ int prepare_program(cih_canvas *cv) {
             return fail(CIH_ERROR_MALFORMED, "malformed draw packet at byte " +
                                                  std::to_string(at));
         desc.canvas = 0;
+        // clip masks: a draw may use mask 0 or one a clip has defined
+        if (desc.kind == CIH_DRAW_CLIP) {
+            if ((s = ensure_mask(cv, (uint32_t)desc.image)))
+                return s;
+        }
+        if (desc.clip_mask && (cv->mask_defined.empty() || !cv->mask_defined[desc.clip_mask]))
+            return fail(CIH_ERROR_MALFORMED, "a draw uses a clip mask no clip has defined");
+        if (desc.kind == CIH_DRAW_CLIP)
+            cv->mask_defined[(uint32_t)desc.image] = true;
         P.descs.push_back(desc);
         at += used;
     }
@@ -561,6 +636,7 @@ int run_program(cih_canvas *cv, bool check) {
         cd.height = cv->height;
         cd.first_draw = 0;
         cd.ndraws = (int32_t)sp.count;
+        cd.masks = (uint64_t)(uintptr_t)cv->mask_table.ptr;
         in.canvases.push_back(cd);
         if ((s = launch(cv->device, in, check)))
             return s;
@@ -711,6 +787,9 @@ int cih_canvas_destroy(cih_canvas *c) {
     c->bitmap.release();
     c->program.payload.release();
     c->program.draws.release();
+    c->mask_table.release();
+    for (dbuf &m : c->mask_store)
+        m.release();
     delete c;
     return CIH_OK;
 }
@@ -939,6 +1018,7 @@ int cih_canvas_stats(cih_canvas *c, uint64_t *stats) {
         cd.height = c->height;
         cd.first_draw = 0;
         cd.ndraws = (int32_t)sp.count;
+        cd.masks = (uint64_t)(uintptr_t)c->mask_table.ptr;
         in.canvases.push_back(cd);
         if ((s = launch(d, in, true)))
             return s;
@@ -1309,6 +1389,11 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
                     destroy_batch(b);
                     return fail(CIH_ERROR_MALFORMED, "malformed draw packet in batch");
                 }
+                if (desc.kind == CIH_DRAW_CLIP || desc.clip_mask) {
+                    destroy_batch(b);
+                    return fail(CIH_ERROR_INVALID, "clip() is supported on single canvases, "
+                                                   "not in batches");
+                }
                 desc.canvas = i;
                 b->descs.push_back(desc);
                 at += used;
@@ -1652,6 +1737,10 @@ void server_worker(cih_server *sv) {
                 size_t used = decode_packet(r->packets.data(), at, r->packets.size(), desc, d);
                 if (!used) {
                     st = CIH_ERROR_MALFORMED;
+                    break;
+                }
+                if (desc.kind == CIH_DRAW_CLIP || desc.clip_mask) { // single canvases only
+                    st = CIH_ERROR_INVALID;
                     break;
                 }
                 r->descs.push_back(desc);

@@ -18,9 +18,10 @@
 // put_image_data) or when the pending list grows large.  Results match the
 // reference's immediate mode because nothing else observes pixels.
 //
+// clip() makes device-side masks that later draws are composited under;
 // is_point_in_path is a host-side query answered by the library without the
 // device.  Not provided (out of scope here): text (set_font, fill_text,
-// stroke_text, measure_text) and clip().  Extension: set_fill_rule.
+// stroke_text, measure_text).  Extension: set_fill_rule.
 #ifndef CANVAS_ITY_HIP_HPP
 #define CANVAS_ITY_HIP_HPP
 
@@ -130,8 +131,8 @@ public:
           shadow_offset_y(0.0f), line_cap(butt), line_join(miter),
           line_dash_offset(0.0f), text_align(start), text_baseline(alphabetic),
           size_x(width), size_y(height), global_alpha(1.0f), shadow_blur(0.0f),
-          line_width(1.0f), miter_limit(10.0f), rule(nonzero), device(0), handle(0),
-          owned(true), saves(0) {
+          line_width(1.0f), miter_limit(10.0f), rule(nonzero), clip_id(0), device(0),
+          handle(0), owned(true), saves(0) {
         reset_state();
         if (width <= 0 || height <= 0)
             return; // state-only instance (save stack entries)
@@ -152,8 +153,8 @@ public:
           shadow_offset_y(0.0f), line_cap(butt), line_join(miter),
           line_dash_offset(0.0f), text_align(start), text_baseline(alphabetic),
           size_x(0), size_y(0), global_alpha(1.0f), shadow_blur(0.0f),
-          line_width(1.0f), miter_limit(10.0f), rule(nonzero), device(0), handle(borrowed),
-          owned(false), saves(0) {
+          line_width(1.0f), miter_limit(10.0f), rule(nonzero), clip_id(0), device(0),
+          handle(borrowed), owned(false), saves(0) {
         reset_state();
         if (handle) {
             cih_canvas_size(handle, &size_x, &size_y);
@@ -419,6 +420,26 @@ public:
     void fill() { emit_path(CIH_DRAW_FILL_PATH, fill_brush); }
     void stroke() { emit_path(CIH_DRAW_STROKE_PATH, stroke_brush); }
     void set_fill_rule(int new_rule) { rule = new_rule == evenodd ? evenodd : nonzero; }
+    // hpp:828, 3057-3099: the current path's fill coverage, multiplied into
+    // the current mask, becomes the new mask.  The device computes it (the
+    // path is rasterised like a fill); the host names it with the lowest id
+    // that no saved state still refers to, so ids are reused.
+    void clip() {
+        uint32_t id = clip_id;
+        for (canvas *s = saves; s; s = s->saves)
+            id = std::max(id, s->clip_id);
+        ++id;
+        std::vector<uint32_t> info(2 * subpaths.size());
+        for (size_t k = 0; k < subpaths.size(); ++k) {
+            info[2 * k] = (uint32_t)subpaths[k].count;
+            info[2 * k + 1] = subpaths[k].closed ? 1u : 0u;
+        }
+        detail::brush none;
+        encode(CIH_DRAW_CLIP, none, CIH_BRUSH_COLOR, points.empty() ? 0 : &points[0],
+               points.size(), info.empty() ? 0 : &info[0], subpaths.size(), inverse, forward,
+               (int32_t)id);
+        clip_id = id;
+    }
     // hpp:843, 3101-3132: the current path's fill flattening and crossing
     // test, evaluated on the host by the library (no device work).
     bool is_point_in_path(float x, float y) {
@@ -557,6 +578,7 @@ private:
     std::vector<detail::xy> points;
     std::vector<detail::subpath> subpaths;
     int rule;
+    uint32_t clip_id; // the current clip mask (0: the whole canvas)
     cih_device *device;
     cih_canvas *handle;
     bool owned;
@@ -576,6 +598,7 @@ private:
     }
 
     static void copy_state(canvas &to, canvas const &from) {
+        to.clip_id = from.clip_id;
         to.global_composite_operation = from.global_composite_operation;
         to.shadow_offset_x = from.shadow_offset_x;
         to.shadow_offset_y = from.shadow_offset_y;
@@ -639,12 +662,14 @@ private:
 
     void encode(uint32_t kind, detail::brush const &b, int brush_kind, detail::xy const *pts,
                 size_t npts, uint32_t const *info, size_t nsub,
-                detail::affine const &paint_inverse, detail::affine const &fwd) {
+                detail::affine const &paint_inverse, detail::affine const &fwd,
+                int32_t new_mask = 0) {
         if (!handle)
             return;
         cih_draw_packet p;
         std::memset(&p, 0, sizeof(p));
         p.kind = kind;
+        p.clip_mask = clip_id;
         p.op = (uint32_t)global_composite_operation;
         p.flags = rule == evenodd && (kind == CIH_DRAW_FILL_PATH) ? CIH_DRAW_FLAG_EVENODD : 0u;
         p.global_alpha = global_alpha;
@@ -661,7 +686,8 @@ private:
         std::memcpy(p.inverse, iv, sizeof(iv));
         std::memcpy(p.paint_inverse, pv, sizeof(pv));
         // render_shadow's early-out (hpp:2398-2401) is decided here.
-        bool shadow = !(shadow_color.a == 0.0f ||
+        bool shadow = kind != CIH_DRAW_CLIP &&
+                      !(shadow_color.a == 0.0f ||
                         (shadow_blur == 0.0f && shadow_offset_x == 0.0f &&
                          shadow_offset_y == 0.0f));
         p.shadow_color[0] = shadow_color.r;
@@ -695,6 +721,10 @@ private:
             p.image_width = b.width;
             p.image_height = b.height;
             p.repetition = b.repetition;
+        }
+        if (kind == CIH_DRAW_CLIP) {
+            colors.clear();
+            p.image = new_mask;
         }
         p.points = (uint32_t)npts;
         p.subpaths = (uint32_t)nsub;

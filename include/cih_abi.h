@@ -62,7 +62,10 @@ enum cih_draw_kind {
     CIH_DRAW_FILL_PATH = 1,   /* fill():   path_to_lines(false) (hpp:3044)   */
     CIH_DRAW_STROKE_PATH = 2, /* stroke(): path_to_lines(true)+stroke_lines  */
     CIH_DRAW_FILL_LINES = 3,  /* fill_rectangle / draw_image polylines       */
-    CIH_DRAW_STROKE_LINES = 4 /* stroke_rectangle polylines + stroke_lines   */
+    CIH_DRAW_STROKE_LINES = 4, /* stroke_rectangle polylines + stroke_lines  */
+    CIH_DRAW_CLIP = 5          /* clip(): the path's fill coverage times the
+                                  mask `clip_mask` becomes mask `image`
+                                  (hpp:3057-3099); paints nothing             */
 };
 
 enum cih_brush_kind {
@@ -93,7 +96,10 @@ typedef struct cih_draw_packet {
     int32_t image;          /* image handle for pattern/image brushes         */
     int32_t image_width, image_height, repetition;
     uint32_t points, subpaths, dashes, colors, stops;
-    uint32_t reserved;
+    uint32_t clip_mask;     /* the clip mask the draw is composited under:
+                               0 = the whole canvas (the constructor's mask,
+                               hpp:2636-2643), k = the mask made by the clip
+                               packet with image = k (ids reused once freed) */
 } cih_draw_packet;
 
 /* ---- device ---------------------------------------------------------------- */

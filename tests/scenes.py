@@ -303,3 +303,95 @@ def kat_hash(get_image_data):
     for byte in data.tolist():
         h = ((((h & 0x1FFFF) << 15) | (h >> 17)) ^ byte) & 0xFFFFFFFF
     return h
+
+
+# ---- clip masks (hpp:3057-3099) ----------------------------------------------
+
+def clip_scenes():
+    """Streams exercising clip(): nested clips under save/restore, every
+    whole-canvas operator under a mask, shadows, gradients and images drawn
+    through a mask, and curved and self-intersecting clip paths."""
+    out = {}
+    # nested clips with save/restore and a curved clip path
+    s = Stream(96, 80)
+    s.set_color(0, 0.9, 0.9, 0.9, 1.0)
+    s.fill_rectangle(0, 0, 96, 80)
+    s.save()
+    s.begin_path(); s.arc(48, 40, 30, 0.0, 6.2831853); s.clip()
+    s.set_color(0, 0.8, 0.2, 0.1, 0.9)
+    s.fill_rectangle(5, 5, 70, 50)
+    s.save()
+    s.begin_path(); s.rectangle(30.5, 10.25, 50, 35.5); s.clip()
+    s.set_linear_gradient(0, 10, 10, 80, 60)
+    s.add_color_stop(0, 0.0, 0.1, 0.2, 0.9, 1.0)
+    s.add_color_stop(0, 1.0, 0.9, 0.8, 0.1, 0.6)
+    s.fill_rectangle(0, 0, 96, 80)
+    s.restore()
+    s.set_color(0, 0.1, 0.7, 0.2, 0.7)
+    s.begin_path(); s.move_to(10, 70); s.line_to(90, 5); s.line_to(60, 78); s.close_path()
+    s.fill()
+    s.restore()
+    s.set_color(1, 0.0, 0.0, 0.0, 1.0)
+    s.set_line_width(3.0)
+    s.stroke_rectangle(8, 8, 40, 30)
+    out["nested"] = s.tobytes()
+    # whole-canvas operators under a mask
+    for op in ("source_copy", "source_in", "destination_in", "destination_atop", "source_out"):
+        s = Stream(64, 48)
+        s.set_color(0, 0.2, 0.3, 0.9, 0.8)
+        s.fill_rectangle(4, 4, 50, 36)
+        s.begin_path(); s.move_to(10, 2); s.bezier_curve_to(70, 10, -10, 40, 55, 46)
+        s.line_to(30, 20); s.close_path(); s.clip()
+        s.set_composite_op(COMPOSITE[op])
+        s.set_color(0, 0.9, 0.5, 0.1, 0.6)
+        s.begin_path(); s.arc(32, 24, 15.5, 0.3, 5.9); s.fill()
+        out["op_" + op] = s.tobytes()
+    # a shadowed draw and an image through a mask
+    s = Stream(80, 64)
+    s.begin_path(); s.rectangle(12.5, 8.5, 50, 44); s.clip()
+    s.set_shadow_color(0.1, 0.1, 0.3, 0.8)
+    s.set_shadow_blur(6.0)
+    s.set_shadow_offset(5.0, 4.0)
+    s.set_color(0, 0.9, 0.8, 0.2, 1.0)
+    s.fill_rectangle(20, 15, 30, 20)
+    s.set_shadow_color(0.0, 0.0, 0.0, 0.0)
+    s.define_image(1, noise_image(24, 20, seed=9))
+    s.draw_image(1, 30, 28, 48, 40)
+    out["shadow_image"] = s.tobytes()
+    # a self-intersecting clip path (non-zero winding 2 inside) and a clip
+    # that leaves nothing
+    s = Stream(64, 64)
+    s.begin_path()
+    s.move_to(32, 2); s.line_to(50, 60); s.line_to(2, 22); s.line_to(62, 22)
+    s.line_to(14, 60); s.close_path(); s.clip()
+    s.set_color(0, 0.3, 0.8, 0.6, 1.0)
+    s.fill_rectangle(0, 0, 64, 64)
+    s.save()
+    s.begin_path(); s.rectangle(100, 100, 10, 10); s.clip()
+    s.set_color(0, 1.0, 0.0, 0.0, 1.0)
+    s.fill_rectangle(0, 0, 64, 64)
+    s.restore()
+    out["star_empty"] = s.tobytes()
+    return out
+
+
+def clipped_fill(clips, size=32):
+    """Clips to each (x, y, w, h) rectangle in turn, then fills the whole
+    canvas opaque white: a pixel's alpha is the product of its overlaps with
+    the clip rectangles (visibility = min(|coverage|, 1) multiplied per clip,
+    and source_over onto transparent black leaves alpha = visibility)."""
+    s = Stream(size, size)
+    for rect in clips:
+        s.begin_path()
+        s.rectangle(*rect)
+        s.clip()
+    s.set_color(0, 1.0, 1.0, 1.0, 1.0)
+    s.fill_rectangle(0, 0, size, size)
+    return s.tobytes()
+
+
+def expected_clipped_alpha(clips, size=32):
+    alpha = np.ones((size, size))
+    for rect in clips:
+        alpha = alpha * expected_rect_alpha(size, rect)
+    return alpha

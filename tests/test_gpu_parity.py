@@ -115,6 +115,27 @@ def test_even_odd_extension(device):
 
 # ---- benchmark configurations --------------------------------------------------------
 
+# ---- clip masks -------------------------------------------------------------------
+
+@pytest.mark.parametrize("name", sorted(scenes.clip_scenes()))
+def test_clip_scene(device, name):
+    check(scenes.clip_scenes()[name], name)
+
+
+@pytest.mark.parametrize("n", [1, 2])
+def test_clip_closed_form(device, n):
+    clips = [(3.25, 5.6, 17.3, 9.15), (8.0, 2.5, 12.75, 20.0)][:n]
+    _, lin = gpu(scenes.clipped_fill(clips))
+    exp = scenes.expected_clipped_alpha(clips)
+    assert np.abs(lin[..., 3] - exp).max() < 4e-6
+    assert np.all(lin[exp == 0.0] == 0.0)
+
+
+def test_clip_refused_in_batches(device):
+    with pytest.raises(cia.CihError):
+        cia.Batch([scenes.clipped_fill([(2.0, 2.0, 10.0, 10.0)])], device)
+
+
 def test_plain_512_full(device):
     m = check(cia.scene(1, 0xC1), "plain-512")
     print("plain-512", m)

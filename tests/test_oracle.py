@@ -157,3 +157,28 @@ def test_malformed_stream_rejected():
     c = oracle.Canvas(8, 8, "port")
     with pytest.raises(ValueError):
         c.replay(bytes(stream))
+
+
+# ---- clip masks (hpp:3057-3099) ------------------------------------------------
+
+CLIP_RECTS = [(3.25, 5.6, 17.3, 9.15), (8.0, 2.5, 12.75, 20.0)]
+
+
+@pytest.mark.skipif(not oracle.available("reference"), reason="reference build absent")
+@pytest.mark.parametrize("name", sorted(scenes.clip_scenes()))
+def test_clip_oracle_matches_reference(name):
+    stream = scenes.clip_scenes()[name]
+    r8, rl = render(stream, "reference")
+    p8, pl = render(stream, "port")
+    assert np.array_equal(rl.view(np.uint32), pl.view(np.uint32))
+    assert np.array_equal(r8, p8)
+
+
+@pytest.mark.parametrize("n", [1, 2])
+def test_clip_closed_form(n):
+    clips = CLIP_RECTS[:n]
+    _, lin = render(scenes.clipped_fill(clips))
+    exp = scenes.expected_clipped_alpha(clips)
+    assert np.abs(lin[..., 3] - exp).max() < 4e-6
+    assert np.all(lin[exp == 0.0] == 0.0)  # nothing outside the mask
+
