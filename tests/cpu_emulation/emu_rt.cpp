@@ -1,0 +1,261 @@
+// Fiber-based SIMT emulation of HIP kernels on the CPU (lanes as coroutines,
+// wave intrinsics as rendezvous), plus a malloc-backed HIP runtime.  Built with
+// AddressSanitizer so that every kernel access outside an allocation is caught.
+#include "hip/hip_runtime.h"
+#include <csetjmp>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ucontext.h>
+#include <vector>
+#include <mutex>
+#include <sanitizer/asan_interface.h>
+#include <sanitizer/common_interface_defs.h>
+
+namespace emu {
+
+struct fiber {
+    ucontext_t uctx;
+    jmp_buf jb;
+    bool started = false, done = false;
+    idx3 tid;
+    int wave = 0, lane = 0;
+    char *stack = nullptr;
+    void *fake = nullptr;
+};
+
+struct rendezvous {
+    int live = 0, arrived = 0, site = 0;
+    const char *what = "";
+    uint64_t gen = 0;
+    uint64_t slots[64], result[64];
+    bool exited[64];
+};
+
+static const size_t kStack = 512 * 1024;
+static std::vector<fiber> fibers;
+static fiber *cur = nullptr;
+static jmp_buf sched_jb;
+static void *sched_fake = nullptr;
+static const void *sched_bottom = nullptr;
+static size_t sched_size = 0;
+static rendezvous waves[16];
+static rendezvous block;
+static idx3 g_block, g_bdim, g_gdim;
+static void (*g_body)(void *) = nullptr;
+static void *g_ctx = nullptr;
+static const char *g_name = "";
+static unsigned long long progress = 0;
+static thread_local idx3 host_idx = {0, 0, 0};
+
+idx3 &thread_idx() { return cur ? cur->tid : host_idx; }
+idx3 &block_idx() { return g_block; }
+idx3 &block_dim() { return g_bdim; }
+idx3 &grid_dim() { return g_gdim; }
+int lane() { return cur->lane; }
+
+void fail(const char *msg) {
+    fprintf(stderr, "EMU FAIL in %s block (%u,%u,%u) thread %u: %s\n", g_name, g_block.x, g_block.y,
+            g_block.z, cur ? cur->tid.x : 0u, msg);
+    abort();
+}
+
+static void to_scheduler() {
+    fiber *f = cur;
+    if (!_setjmp(f->jb)) {
+        __sanitizer_start_switch_fiber(&f->fake, sched_bottom, sched_size);
+        _longjmp(sched_jb, 1);
+    }
+    __sanitizer_finish_switch_fiber(f->fake, nullptr, nullptr);
+}
+
+static void complete(rendezvous &r) {
+    for (int l = 0; l < 64; ++l)
+        r.result[l] = r.exited[l] ? 0 : r.slots[l];
+    r.arrived = 0;
+    r.gen++;
+    progress++;
+}
+
+const uint64_t *wave_exchange(uint64_t v, int site, const char *what) {
+    fiber *f = cur;
+    rendezvous &w = waves[f->wave];
+    if (w.arrived == 0) {
+        w.site = site;
+        w.what = what;
+    } else if (w.site != site) {
+        char buf[256];
+        snprintf(buf, sizeof buf, "divergent wave op: line %d (%s) vs line %d (%s)", site, what,
+                 w.site, w.what);
+        fail(buf);
+    }
+    w.slots[f->lane] = v;
+    w.arrived++;
+    progress++;
+    uint64_t my = w.gen;
+    if (w.arrived == w.live)
+        complete(w);
+    while (w.gen == my)
+        to_scheduler();
+    return w.result;
+}
+
+void wave_barrier(int site) { wave_exchange(0, site, "wave_sync"); }
+
+void block_barrier(int site) {
+    if (block.arrived == 0)
+        block.site = site;
+    else if (block.site != site) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "divergent __syncthreads: line %d vs line %d", site, block.site);
+        fail(buf);
+    }
+    block.arrived++;
+    progress++;
+    uint64_t my = block.gen;
+    if (block.arrived == block.live)
+        complete(block);
+    while (block.gen == my)
+        to_scheduler();
+}
+
+static void fiber_entry() {
+    __sanitizer_finish_switch_fiber(nullptr, &sched_bottom, &sched_size);
+    fiber *f = cur;
+    g_body(g_ctx);
+    f->done = true;
+    progress++;
+    rendezvous &w = waves[f->wave];
+    w.live--;
+    w.exited[f->lane] = true;
+    if (w.arrived && w.arrived == w.live)
+        complete(w);
+    block.live--;
+    if (block.arrived && block.arrived == block.live)
+        complete(block);
+    __sanitizer_start_switch_fiber(nullptr, sched_bottom, sched_size);
+    _longjmp(sched_jb, 1);
+}
+
+static void run_block(unsigned bx, unsigned by, unsigned bz) {
+    g_block = {bx, by, bz};
+    unsigned n = g_bdim.x * g_bdim.y * g_bdim.z;
+    if (fibers.size() < n)
+        fibers.resize(n);
+    int nwaves = (int)((n + 63) / 64);
+    for (int k = 0; k < nwaves; ++k) {
+        waves[k] = rendezvous();
+        waves[k].live = (int)(n - 64 * k < 64 ? n - 64 * k : 64);
+        for (int l = 0; l < 64; ++l)
+            waves[k].exited[l] = l >= waves[k].live;
+    }
+    block = rendezvous();
+    block.live = (int)n;
+    for (int l = 0; l < 64; ++l)
+        block.exited[l] = false;
+    for (unsigned t = 0; t < n; ++t) {
+        fiber &f = fibers[t];
+        if (!f.stack)
+            f.stack = (char *)malloc(kStack);
+        f.started = false;
+        f.done = false;
+        f.tid = {t % g_bdim.x, (t / g_bdim.x) % g_bdim.y, t / (g_bdim.x * g_bdim.y)};
+        f.wave = (int)(t / 64);
+        f.lane = (int)(t % 64);
+        f.fake = nullptr;
+    }
+    unsigned remaining = n;
+    while (remaining) {
+        unsigned long long before = progress;
+        for (unsigned t = 0; t < n; ++t) {
+            fiber &f = fibers[t];
+            if (f.done)
+                continue;
+            cur = &f;
+            if (!_setjmp(sched_jb)) {
+                if (!f.started) {
+                    f.started = true;
+                    getcontext(&f.uctx);
+                    f.uctx.uc_stack.ss_sp = f.stack;
+                    f.uctx.uc_stack.ss_size = kStack;
+                    f.uctx.uc_link = nullptr;
+                    makecontext(&f.uctx, fiber_entry, 0);
+                    __sanitizer_start_switch_fiber(&sched_fake, f.stack, kStack);
+                    ucontext_t dummy;
+                    swapcontext(&dummy, &f.uctx);
+                } else {
+                    __sanitizer_start_switch_fiber(&sched_fake, f.stack, kStack);
+                    _longjmp(f.jb, 1);
+                }
+            }
+            __sanitizer_finish_switch_fiber(sched_fake, nullptr, nullptr);
+            cur = nullptr;
+            if (f.done)
+                remaining--;
+        }
+        if (progress == before && remaining) {
+            char buf[160];
+            snprintf(buf, sizeof buf, "deadlock: %u lanes blocked (block barrier %d arrived, wave0 %d arrived at line %d)",
+                     remaining, block.arrived, waves[0].arrived, waves[0].site);
+            fail(buf);
+        }
+    }
+}
+
+unsigned long long kernels_run = 0;
+
+static std::recursive_mutex run_mutex; // server threads launch kernels too
+
+void run_kernel(const char *name, dim3 grid, dim3 blk, void (*body)(void *), void *ctx) {
+    std::lock_guard<std::recursive_mutex> lock(run_mutex);
+    g_name = name;
+    g_body = body;
+    g_ctx = ctx;
+    g_bdim = {blk.x, blk.y, blk.z};
+    g_gdim = {grid.x, grid.y, grid.z};
+    if (blk.x * blk.y * blk.z > 1024)
+        fail("block too large");
+    for (unsigned z = 0; z < grid.z; ++z)
+        for (unsigned y = 0; y < grid.y; ++y)
+            for (unsigned x = 0; x < grid.x; ++x)
+                run_block(x, y, z);
+    kernels_run++;
+}
+
+} // namespace emu
+
+// ---- runtime --------------------------------------------------------------------
+struct emu_stream { int id; };
+struct emu_event { int id; };
+static int g_err = hipSuccess;
+
+hipError_t hipMalloc(void **p, size_t n) {
+    *p = malloc(n ? n : 1);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipFree(void *p) { free(p); return hipSuccess; }
+hipError_t hipHostMalloc(void **p, size_t n, unsigned) { return hipMalloc(p, n); }
+hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { if (n) memmove(d, s, n); return hipSuccess; }
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t) { return hipMemcpy(d, s, n, k); }
+hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamCreate(hipStream_t *s) { *s = new emu_stream{1}; return hipSuccess; }
+hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { return hipStreamCreate(s); }
+hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
+hipError_t hipGetLastError() { int e = g_err; g_err = hipSuccess; return e; }
+const char *hipGetErrorString(hipError_t) { return "emulated error"; }
+hipError_t hipSetDevice(int) { return hipSuccess; }
+hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
+hipError_t hipEventCreate(hipEvent_t *e) { *e = new emu_event{1}; return hipSuccess; }
+hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
+hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.0f; return hipSuccess; }
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorNotSupported; }
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *) { return hipErrorNotSupported; }
+hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, void *, char *, size_t) { return hipErrorNotSupported; }
+hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }

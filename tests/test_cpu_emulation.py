@@ -1,0 +1,56 @@
+"""The kernel source run on the CPU under AddressSanitizer.
+
+tests/cpu_emulation builds the unmodified kernels (cih_kernels.hip) and
+runtime against an emulated HIP runtime -- each workgroup's lanes as
+coroutines meeting at every wave intrinsic and barrier, device buffers as
+exactly-sized host allocations, each checked frame run again with arenas
+exactly as large as it asked for -- so that any kernel access outside an
+allocation aborts with AddressSanitizer's report, and the emulated renders
+must meet the parity bar against the oracle.  This is a CPU check of the GPU
+code's memory safety and arithmetic; the GPU tests are the parity tests of
+the product itself.  Skipped when g++ with AddressSanitizer is unavailable.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "cpu_emulation"))
+
+import build as emulation  # noqa: E402
+
+pytestmark = pytest.mark.skipif(shutil.which("g++") is None, reason="g++ is not available")
+
+
+@pytest.fixture(scope="module")
+def library():
+    try:
+        emulation.asan_runtime()
+    except (OSError, subprocess.CalledProcessError):
+        pytest.skip("AddressSanitizer runtime not found")
+    return emulation.build()
+
+
+def run(library, names):
+    env = dict(os.environ)
+    env["LD_PRELOAD"] = emulation.asan_runtime()
+    env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=1"
+    p = subprocess.run([sys.executable, os.path.join(HERE, "cpu_emulation", "run_scenes.py"),
+                        library] + names, env=env, capture_output=True, text=True,
+                       timeout=900)
+    report = (p.stdout + p.stderr)[-6000:]
+    assert p.returncode == 0, report
+    for n in names:
+        assert "%s ok" % n in p.stdout, report
+
+
+def test_configurations_memory_safe_and_in_parity(library):
+    run(library, ["plain_small", "large_small", "effects_small"])
+
+
+def test_clip_masks_memory_safe_and_in_parity(library):
+    run(library, ["clip_nested", "clip_op_destination_in", "clip_shadow_image",
+                  "clip_star_empty"])
