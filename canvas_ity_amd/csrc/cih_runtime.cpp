@@ -596,12 +596,23 @@ int prepare_program(cih_canvas *cv) {
     HIP_TRY(hipMemcpyAsync(P.draws.ptr, stage + bytes.size(), sizeof(draw_desc) * P.descs.size(),
                            hipMemcpyHostToDevice, d->stream));
     // Frames of bounded size: shadow images are the largest intermediates.
+    // A frame also ends before a clip that redefines a mask id the frame
+    // already uses: a frame builds all its masks before compositing, so each
+    // id must name one mask within it.
     const uint64_t shadow_budget = (uint64_t)1 << 30; // floats
+    std::vector<unsigned char> used(kMaxMasks);
     size_t first = 0;
     while (first < P.descs.size()) {
         size_t last = first;
         uint64_t shadow = 0;
+        std::fill(used.begin(), used.end(), (unsigned char)0);
         while (last < P.descs.size() && last - first < 8192) {
+            const draw_desc &dd = P.descs[last];
+            if (dd.kind == CIH_DRAW_CLIP && used[(uint32_t)dd.image] && last > first)
+                break;
+            used[(uint32_t)dd.clip_mask] = 1;
+            if (dd.kind == CIH_DRAW_CLIP)
+                used[(uint32_t)dd.image] = 1;
             uint64_t need = 0;
             if (P.descs[last].shadow) {
                 int b = P.descs[last].border;

@@ -27,6 +27,7 @@ def streams():
         "effects_small": cia.scene(3, 0xC3, 96, 96, 10, 0),
     }
     out.update({"clip_" + k: v for k, v in scenes.clip_scenes().items()})
+    out.update({"reference_" + k: v for k, v in scenes.reference_clip_tests(64).items()})
     return out
 
 

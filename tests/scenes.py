@@ -395,3 +395,95 @@ def expected_clipped_alpha(clips, size=32):
     for rect in clips:
         alpha = alpha * expected_rect_alpha(size, rect)
     return alpha
+
+
+def reference_clip_tests(size=256):
+    """The call sequences of the reference's own clip-dependent tests
+    (test.cpp:440-467 global_composite_operation, 551-577
+    shadow_blur_composite, 1396-1418 clip, 1420-1447 clip_winding, 1855-1873
+    save_restore) at the test suite's 256x256, as command streams."""
+    w = h = float(size)
+    out = {}
+    s = Stream(size, size)
+    ops = [("source_in", 1), ("source_copy", 2), ("source_out", 3), ("destination_in", 4),
+           ("destination_atop", 7), ("lighter", 10), ("destination_over", 11),
+           ("destination_out", 12), ("source_atop", 13), ("source_over", 14),
+           ("exclusive_or", 15)]
+    bw, bh = 0.25 * w, 0.25 * h
+    for _, op in ops:
+        col, row = float(op % 4), float(op // 4)
+        s.save()
+        s.begin_path(); s.rectangle(col * bw, row * bh, bw, bh); s.clip()
+        s.set_color(0, 0.0, 0.0, 1.0, 1.0)
+        s.fill_rectangle((col + 0.4) * bw, (row + 0.4) * bh, 0.4 * bw, 0.4 * bh)
+        s.set_composite_op(op)
+        s.set_color(0, 1.0, 0.0, 0.0, 1.0)
+        s.fill_rectangle((col + 0.2) * bw, (row + 0.2) * bh, 0.4 * bw, 0.4 * bh)
+        s.restore()
+    out["global_composite_operation"] = s.tobytes()
+
+    s = Stream(size, size)
+    radius = min(w, h) * 0.5
+    s.arc(0.5 * w, 0.5 * h, radius, 0.0, 6.28318531); s.clip()
+    s.set_color(0, 0.0, 0.0, 1.0, 1.0)
+    s.set_shadow_color(0.0, 0.0, 0.0, 1.0)
+    s.fill_rectangle(0.4 * w, 0.0, 0.2 * w, h)
+    s.set_composite_op(7)  # destination_atop
+    s.set_color(1, 1.0, 0.0, 0.0, 1.0)
+    s.set_line_dash([16.0, 4.0])
+    s.set_line_width(15.0)
+    s.set_shadow_offset(5.0, 5.0)
+    s.set_shadow_blur(6.0)
+    s.set_shadow_color(0.0, 0.0, 0.0, 1.0)
+    s.begin_path(); s.arc(0.45 * w, 0.85 * h, radius * 0.5, 0.0, 6.28318531); s.close_path()
+    s.stroke()
+    s.set_composite_op(14)  # source_over
+    s.begin_path(); s.arc(0.75 * w, 0.25 * h, radius, 0.0, 6.28318531); s.close_path()
+    s.stroke()
+    out["shadow_blur_composite"] = s.tobytes()
+
+    s = Stream(size, size)
+    s.set_line_width(8.0)
+    for step in range(8):
+        angle = np.float32(np.float32(step) / np.float32(8.0)) * np.float32(6.28318531)
+        s.set_color(1, 0.0, float(step & 1), 0.0, 1.0)
+        s.begin_path()
+        s.arc(0.5 * w + 0.8 * radius * float(np.cos(angle, dtype=np.float32)),
+              0.5 * h + 0.8 * radius * float(np.sin(angle, dtype=np.float32)),
+              radius, 0.0, 6.28318531)
+        s.close_path(); s.stroke(); s.clip()
+    s.begin_path(); s.clip()
+    s.set_color(0, 1.0, 0.0, 1.0, 1.0)
+    s.fill_rectangle(0.0, 0.0, w, h)
+    out["clip"] = s.tobytes()
+
+    s = Stream(size, size)
+    for x, y in ((0.125, 0.125), (0.625, 0.125), (0.625, 0.625), (0.125, 0.625)):
+        (s.move_to if (x, y) == (0.125, 0.125) else s.line_to)(x * w, y * h)
+    s.move_to(0.25 * w, 0.25 * h); s.line_to(0.75 * w, 0.25 * h)
+    s.line_to(0.75 * w, 0.75 * h); s.line_to(0.25 * w, 0.75 * h)
+    s.move_to(0.375 * w, 0.375 * h); s.line_to(0.375 * w, 0.875 * h)
+    s.line_to(0.875 * w, 0.875 * h); s.line_to(0.875 * w, 0.375 * h)
+    s.set_color(0, 1.0, 0.0, 0.0, 1.0)
+    s.fill(); s.clip()
+    s.set_line_width(4.0); s.stroke()
+    s.set_line_width(6.0)
+    s.begin_path()
+    for step in range(32):
+        s.move_to(step / 16.0 * w, 0.0); s.line_to(step / 16.0 * w - w, h)
+    s.stroke()
+    out["clip_winding"] = s.tobytes()
+
+    s = Stream(size, size)
+    s.rectangle(w * 0.25, h * 0.25, w * 0.25, h * 0.25)
+    s.set_color(1, 0.0, 0.0, 1.0, 1.0)
+    s.set_line_width(8.0)
+    s.save(); s.clip()
+    s.begin_path(); s.rectangle(w * 0.25, h * 0.25, w * 0.5, h * 0.5)
+    s.set_color(1, 1.0, 0.0, 0.0, 1.0)
+    s.set_line_width(1.0)
+    s.restore(); s.restore()
+    s.stroke()
+    s.save(); s.save()
+    out["save_restore"] = s.tobytes()
+    return out

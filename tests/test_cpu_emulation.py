@@ -53,4 +53,4 @@ def test_configurations_memory_safe_and_in_parity(library):
 
 def test_clip_masks_memory_safe_and_in_parity(library):
     run(library, ["clip_nested", "clip_op_destination_in", "clip_shadow_image",
-                  "clip_star_empty"])
+                  "clip_star_empty", "reference_global_composite_operation"])

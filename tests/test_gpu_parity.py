@@ -131,6 +131,12 @@ def test_clip_closed_form(device, n):
     assert np.all(lin[exp == 0.0] == 0.0)
 
 
+@pytest.mark.parametrize("name", sorted(scenes.reference_clip_tests()))
+def test_reference_clip_test(device, name):
+    # the reference's clip-dependent tests (test.cpp) as command streams
+    check(scenes.reference_clip_tests()[name], name)
+
+
 def test_clip_refused_in_batches(device):
     with pytest.raises(cia.CihError):
         cia.Batch([scenes.clipped_fill([(2.0, 2.0, 10.0, 10.0)])], device)

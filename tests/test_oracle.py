@@ -182,3 +182,13 @@ def test_clip_closed_form(n):
     assert np.abs(lin[..., 3] - exp).max() < 4e-6
     assert np.all(lin[exp == 0.0] == 0.0)  # nothing outside the mask
 
+
+@pytest.mark.skipif(not oracle.available("reference"), reason="reference build absent")
+@pytest.mark.parametrize("name", sorted(scenes.reference_clip_tests()))
+def test_reference_clip_tests_oracle(name):
+    stream = scenes.reference_clip_tests()[name]
+    r8, rl = render(stream, "reference")
+    p8, pl = render(stream, "port")
+    assert np.array_equal(rl.view(np.uint32), pl.view(np.uint32))
+    assert np.array_equal(r8, p8)
+
