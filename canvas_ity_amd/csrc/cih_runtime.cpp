@@ -554,6 +554,45 @@ int ensure_mask(cih_canvas *cv, uint32_t id) {
     return CIH_OK;
 }
 
+void release_masks(cih_canvas *cv) {
+    cv->mask_table.release();
+    for (dbuf &m : cv->mask_store)
+        m.release();
+    cv->mask_store.clear();
+    cv->mask_defined.clear();
+    cv->mask_bytes = 0;
+}
+
+// The clip-mask bookkeeping of one decoded draw, in submission order: a clip
+// gets storage for the id it defines; any draw may use mask 0 or a defined id.
+int note_masks(cih_canvas *cv, const draw_desc &desc) {
+    int s;
+    if (desc.kind == CIH_DRAW_CLIP && (s = ensure_mask(cv, (uint32_t)desc.image)))
+        return s;
+    if (desc.clip_mask && (cv->mask_defined.empty() || !cv->mask_defined[desc.clip_mask]))
+        return fail(CIH_ERROR_MALFORMED, "a draw uses a clip mask no clip has defined");
+    if (desc.kind == CIH_DRAW_CLIP)
+        cv->mask_defined[(uint32_t)desc.image] = true;
+    return CIH_OK;
+}
+
+// Whether a frame must end before draw `dd` of canvas `canvas`: a clip that
+// redefines a mask id the frame already uses on that canvas (a frame builds
+// all its masks before compositing, so each id must name one mask within
+// it).  `used` and `current` track the frame; they reset at a new canvas.
+bool mask_cut(const draw_desc &dd, int canvas, std::vector<unsigned char> &used, int &current) {
+    if (canvas != current) {
+        std::fill(used.begin(), used.end(), (unsigned char)0);
+        current = canvas;
+    }
+    if (dd.kind == CIH_DRAW_CLIP && used[(uint32_t)dd.image])
+        return true;
+    used[(uint32_t)dd.clip_mask] = 1;
+    if (dd.kind == CIH_DRAW_CLIP)
+        used[(uint32_t)dd.image] = 1;
+    return false;
+}
+
 // Decodes pending packets into the canvas' resident program (payload and
 // draw records in HBM) split into frames of bounded size.
 int prepare_program(cih_canvas *cv) {
@@ -572,15 +611,8 @@ int prepare_program(cih_canvas *cv) {
             return fail(CIH_ERROR_MALFORMED, "malformed draw packet at byte " +
                                                  std::to_string(at));
         desc.canvas = 0;
-        // clip masks: a draw may use mask 0 or one a clip has defined
-        if (desc.kind == CIH_DRAW_CLIP) {
-            if ((s = ensure_mask(cv, (uint32_t)desc.image)))
-                return s;
-        }
-        if (desc.clip_mask && (cv->mask_defined.empty() || !cv->mask_defined[desc.clip_mask]))
-            return fail(CIH_ERROR_MALFORMED, "a draw uses a clip mask no clip has defined");
-        if (desc.kind == CIH_DRAW_CLIP)
-            cv->mask_defined[(uint32_t)desc.image] = true;
+        if ((s = note_masks(cv, desc)))
+            return s;
         P.descs.push_back(desc);
         at += used;
     }
@@ -605,14 +637,10 @@ int prepare_program(cih_canvas *cv) {
     while (first < P.descs.size()) {
         size_t last = first;
         uint64_t shadow = 0;
-        std::fill(used.begin(), used.end(), (unsigned char)0);
+        int current = -1;
         while (last < P.descs.size() && last - first < 8192) {
-            const draw_desc &dd = P.descs[last];
-            if (dd.kind == CIH_DRAW_CLIP && used[(uint32_t)dd.image] && last > first)
+            if (mask_cut(P.descs[last], 0, used, current) && last > first)
                 break;
-            used[(uint32_t)dd.clip_mask] = 1;
-            if (dd.kind == CIH_DRAW_CLIP)
-                used[(uint32_t)dd.image] = 1;
             uint64_t need = 0;
             if (P.descs[last].shadow) {
                 int b = P.descs[last].border;
@@ -798,9 +826,7 @@ int cih_canvas_destroy(cih_canvas *c) {
     c->bitmap.release();
     c->program.payload.release();
     c->program.draws.release();
-    c->mask_table.release();
-    for (dbuf &m : c->mask_store)
-        m.release();
+    release_masks(c);
     delete c;
     return CIH_OK;
 }
@@ -1273,8 +1299,10 @@ void drop_graph(cih_batch *b) {
 
 void destroy_batch(cih_batch *b) {
     drop_graph(b);
-    for (cih_canvas *v : b->views)
+    for (cih_canvas *v : b->views) {
+        release_masks(v);
         delete v;
+    }
     dbuf *all[] = {&b->bitmaps, &b->all_canvases, &b->prefix_dev, &b->srgb, &b->payload,
                    &b->draws, &b->frame_canvases, &b->frame_items};
     for (dbuf *x : all)
@@ -1400,10 +1428,9 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
                     destroy_batch(b);
                     return fail(CIH_ERROR_MALFORMED, "malformed draw packet in batch");
                 }
-                if (desc.kind == CIH_DRAW_CLIP || desc.clip_mask) {
+                if ((s = note_masks(v, desc))) {
                     destroy_batch(b);
-                    return fail(CIH_ERROR_INVALID, "clip() is supported on single canvases, "
-                                                   "not in batches");
+                    return s;
                 }
                 desc.canvas = i;
                 b->descs.push_back(desc);
@@ -1412,17 +1439,24 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
             std::vector<unsigned char>().swap(v->pending);
             first_draw[(size_t)i + 1] = b->descs.size();
         }
+        // each canvas' clip masks (allocated while decoding)
+        for (int i = 0; i < count; ++i)
+            table[(size_t)i].masks = (uint64_t)(uintptr_t)b->views[(size_t)i]->mask_table.ptr;
         // frames: consecutive draw ranges of bounded size; canvas lists per frame
         const size_t max_draws = 32768;
         const uint64_t shadow_budget = (uint64_t)1 << 30;
         std::vector<canvas_desc> frame_canvases;
         std::vector<uint32_t> frame_items;
+        std::vector<unsigned char> used(kMaxMasks);
         size_t draw = 0;
         while (draw < b->descs.size()) {
             size_t last = draw;
             uint64_t shadow = 0;
+            int current = -1;
             while (last < b->descs.size() && last - draw < max_draws) {
                 const draw_desc &dd = b->descs[last];
+                if (mask_cut(dd, dd.canvas, used, current) && last > draw)
+                    break;
                 uint64_t need = 0;
                 if (dd.shadow) {
                     const cih_canvas *v = b->views[(size_t)dd.canvas];
@@ -1669,6 +1703,14 @@ struct server_request {
     int width = 0, height = 0, status = CIH_OK;
     uint64_t pool_at = 0, bytes = 0;
     double t_submit = 0, t_done = 0;
+    cih_canvas *masks = nullptr; // the request's clip masks, while it has any
+    void drop_masks() {
+        if (masks) {
+            release_masks(masks);
+            delete masks;
+            masks = nullptr;
+        }
+    }
 };
 
 struct server_batch {
@@ -1734,14 +1776,17 @@ void server_worker(cih_server *sv) {
             r = sv->arrived.front();
             sv->arrived.pop_front();
         }
-        cih_canvas view;
-        view.device = d;
-        view.width = r->width;
-        view.height = r->height;
-        int st = cih_canvas_replay(&view, r->stream.data(), r->stream.size());
+        cih_canvas *view = new (std::nothrow) cih_canvas;
+        int st = view ? CIH_OK : CIH_ERROR_OUT_OF_MEMORY;
+        if (view) {
+            view->device = d;
+            view->width = r->width;
+            view->height = r->height;
+            st = cih_canvas_replay(view, r->stream.data(), r->stream.size());
+        }
         if (st == CIH_OK) {
             std::lock_guard<std::recursive_mutex> dlock(d->mutex); // images table
-            r->packets.swap(view.pending);
+            r->packets.swap(view->pending);
             size_t at = 0;
             while (at < r->packets.size()) {
                 draw_desc desc;
@@ -1750,13 +1795,17 @@ void server_worker(cih_server *sv) {
                     st = CIH_ERROR_MALFORMED;
                     break;
                 }
-                if (desc.kind == CIH_DRAW_CLIP || desc.clip_mask) { // single canvases only
-                    st = CIH_ERROR_INVALID;
+                if ((st = note_masks(view, desc)) != CIH_OK)
                     break;
-                }
                 r->descs.push_back(desc);
                 at += used;
             }
+        }
+        if (view && view->mask_table.ptr && st == CIH_OK) {
+            r->masks = view; // kept for the request's frames
+        } else if (view) {
+            release_masks(view);
+            delete view;
         }
         std::vector<unsigned char>().swap(r->stream);
         std::lock_guard<std::mutex> lock(sv->mu);
@@ -1788,6 +1837,7 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
         table[i].bitmap = (uint64_t)(uintptr_t)((char *)sv->pool.ptr + r->pool_at);
         table[i].width = r->width;
         table[i].height = r->height;
+        table[i].masks = r->masks ? (uint64_t)(uintptr_t)r->masks->mask_table.ptr : 0;
         table[i].first_draw = 0;
         table[i].ndraws = 0;
         uint32_t base = (uint32_t)payload.size();
@@ -1805,11 +1855,15 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
     size_t draw = 0;
     const size_t max_draws = 32768;
     const uint64_t shadow_budget = (uint64_t)1 << 30; // shadow floats per frame
+    std::vector<unsigned char> used(kMaxMasks);
     while (draw < b->descs.size()) {
         size_t last = draw;
         uint64_t shadow = 0;
+        int current = -1;
         while (last < b->descs.size() && last - draw < max_draws) {
             const draw_desc &dd = b->descs[last];
+            if (mask_cut(dd, dd.canvas, used, current) && last > draw)
+                break;
             uint64_t need = 0;
             if (dd.shadow) {
                 const server_request *r = b->requests[(size_t)dd.canvas];
@@ -1947,6 +2001,7 @@ int finish_server_batch(cih_server *sv, int k) {
                          (const unsigned char *)b->srgb_host.ptr + 4 * b->pixel_prefix[i] + bytes);
         sv->allocator.release(r->pool_at, r->bytes);
         r->t_done = t;
+        r->drop_masks();
         std::vector<unsigned char>().swap(r->packets);
         std::vector<draw_desc>().swap(r->descs);
         sv->completed.push_back(r->id);
@@ -1968,6 +2023,7 @@ void abort_server_batch(cih_server *sv, int k, server_batch *b, int status) {
     (void)hipGetLastError();
     for (server_request *r : b->requests) {
         sv->allocator.release(r->pool_at, r->bytes);
+        r->drop_masks();
         r->status = status;
         r->t_done = now_ms();
         sv->completed.push_back(r->id);
@@ -2162,6 +2218,7 @@ int cih_server_result(cih_server *sv, uint64_t request, unsigned char *image, in
         *latency_ms = r->t_done - r->t_submit;
     if (image && status == CIH_OK)
         std::memcpy(image, r->result.data(), r->result.size());
+    r->drop_masks();
     delete r;
     return status == CIH_OK ? CIH_OK : fail(status, "request failed");
 }
@@ -2194,8 +2251,10 @@ int cih_server_stop(cih_server *sv) {
         sv->lanes[k].release();
         hipStreamDestroy(sv->lanes[k].stream);
     }
-    for (auto &kv : sv->requests)
+    for (auto &kv : sv->requests) {
+        kv.second->drop_masks();
         delete kv.second;
+    }
     sv->pool.release();
     delete sv;
     return CIH_OK;

@@ -107,6 +107,9 @@ TIGHT_RERUN = """        if (!hc[COUNTER_OVERFLOW]) {
                     const canvas_desc &cd = canvases_dev[c];
                     std::memcpy((void *)(uintptr_t)cd.bitmap, saved[c].data(), saved[c].size());
                 }
+                keep[0] = F.points_cap, keep[1] = F.words_cap;
+                keep[2] = F.runs_cap, keep[3] = F.shadow_cap;
+                regrow = true;
                 L.points.release(); L.words.release(); L.runs.release();
                 L.steps.release(); L.shadow.release();
                 if ((s = ensure_arenas(L, hc[COUNTER_POINTS], hc[COUNTER_WORDS],
@@ -115,10 +118,16 @@ TIGHT_RERUN = """        if (!hc[COUNTER_OVERFLOW]) {
                 attempt = 6;
                 continue;
             }
+            // arenas back to at least their earlier sizes: later unchecked
+            // runs of other frames rely on arenas that only grow
+            if (regrow && (s = ensure_arenas(L, keep[0], keep[1], keep[2], keep[3])))
+                return s;
             return CIH_OK;
         }"""
 
 SAVE_CANVASES = """    L.last_ndraws = n;
+    uint64_t keep[4] = {0, 0, 0, 0};
+    bool regrow = false;
     std::vector<std::vector<unsigned char>> saved(ncanvases);
     if (check)
         for (size_t c = 0; c < ncanvases; ++c) {

@@ -122,9 +122,12 @@ def test_clip_scene(device, name):
     check(scenes.clip_scenes()[name], name)
 
 
+CLIPS = [(3.25, 5.6, 17.3, 9.15), (8.0, 2.5, 12.75, 20.0)]
+
+
 @pytest.mark.parametrize("n", [1, 2])
 def test_clip_closed_form(device, n):
-    clips = [(3.25, 5.6, 17.3, 9.15), (8.0, 2.5, 12.75, 20.0)][:n]
+    clips = CLIPS[:n]
     _, lin = gpu(scenes.clipped_fill(clips))
     exp = scenes.expected_clipped_alpha(clips)
     assert np.abs(lin[..., 3] - exp).max() < 4e-6
@@ -137,9 +140,28 @@ def test_reference_clip_test(device, name):
     check(scenes.reference_clip_tests()[name], name)
 
 
-def test_clip_refused_in_batches(device):
-    with pytest.raises(cia.CihError):
-        cia.Batch([scenes.clipped_fill([(2.0, 2.0, 10.0, 10.0)])], device)
+def test_clip_in_batches_and_server(device):
+    # every clip scene in one batch and through the server: the same pixels
+    # as each rendered alone (masks are per canvas; frames end where a canvas
+    # redefines a mask id it has used)
+    streams = list(scenes.clip_scenes().values()) + [scenes.clipped_fill(CLIPS)]
+    alone = [cia.render(s, device) for s in streams]
+    b = cia.Batch(streams, device)
+    b.execute(to_host=True)
+    for i in range(len(streams)):
+        assert np.array_equal(alone[i][1].view(np.uint32), b.linear(i).view(np.uint32)), i
+        assert np.array_equal(alone[i][0], b.rgba8(i)), i
+    b.close()
+    server = cia.Server(hbm_budget=1 << 30, max_batch=4, device=device)
+    ids = [server.submit(s) for s in streams]
+    got = {}
+    while len(got) < len(ids):
+        rid = server.wait(timeout_ms=60000)
+        assert rid is not None, "server stalled"
+        got[rid] = server.result(rid)[0]
+    server.stop()
+    for i, rid in enumerate(ids):
+        assert np.array_equal(alone[i][0], got[rid]), i
 
 
 def test_plain_512_full(device):
