@@ -17,6 +17,7 @@ LIBRARY = os.path.join(HERE, "libcanvas_ity_hip.so")
 OK = 0
 ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "device error",
           -4: "malformed input", -5: "over memory budget", -6: "timeout"}
+ERROR_BUDGET = -5
 
 # composite_operation values (canvas_ity.hpp:146-149)
 OPS = {"source_in": 1, "source_copy": 2, "source_out": 3, "destination_in": 4,
@@ -76,6 +77,13 @@ SIGNATURES = {
     "cih_free": (None, [_P]),
     "cih_path_contains": (_c.c_int, [_P, _c.c_size_t, _P, _c.c_size_t, _c.c_float, _c.c_float,
                                      _c.POINTER(_c.c_int)]),
+    "cih_font_check": (_c.c_int, [_P, _c.c_size_t, _c.POINTER(_c.c_int)]),
+    "cih_text_measure": (_c.c_int, [_P, _c.c_size_t, _c.c_float, _c.c_char_p,
+                                    _c.POINTER(_c.c_float)]),
+    "cih_text_outline": (_c.c_int, [_P, _c.c_size_t, _c.c_float, _c.c_char_p, _c.c_float,
+                                    _c.c_float, _c.c_float, _c.c_int, _c.c_int, _P, _c.c_int,
+                                    _c.c_float, _P, _c.c_size_t, _P, _c.c_size_t,
+                                    _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)]),
 }
 
 _lib = None
@@ -131,6 +139,58 @@ def path_contains(points, subpaths, x, y):
                                        sub.ctypes.data_as(_P), sub.shape[0], float(x), float(y),
                                        _c.byref(inside)), "cih_path_contains")
     return bool(inside.value)
+
+
+def _text_bytes(text):
+    return text.encode("utf-8") if isinstance(text, str) else bytes(text)
+
+
+def font_units(font):
+    """Units per em of a TrueType file (bytes), or None if set_font would
+    reject it (cih_font_check)."""
+    units = _c.c_int()
+    if library().cih_font_check(font, len(font), _c.byref(units)) != OK:
+        return None
+    return units.value
+
+
+def measure_text(font, size, text):
+    """measure_text of `text` (str or UTF-8 bytes) in a font (bytes) at `size`,
+    with the reference's scale size / units per em (cih_text_measure)."""
+    units = font_units(font)
+    if units is None:
+        return 0.0
+    scale = np.float32(np.float32(size) / np.float32(units))
+    width = _c.c_float()
+    _check(library().cih_text_measure(font, len(font), float(scale), _text_bytes(text),
+                                      _c.byref(width)), "cih_text_measure")
+    return width.value
+
+
+def text_outline(font, size, text, x, y, maximum_width=1.0e30, align=0, baseline=0,
+                 forward=(1.0, 0.0, 0.0, 1.0, 0.0, 0.0), stroking=False, line_width=1.0):
+    """Glyph outline segments of fill_text/stroke_text (cih_text_outline):
+    (points (4k, 2) float32 -- start, control, control, end per cubic --,
+    subpaths (m, 2) uint32 (count, closed))."""
+    units = font_units(font)
+    if units is None:
+        raise CihError("not a usable TrueType font")
+    scale = np.float32(np.float32(size) / np.float32(units))
+    fw = np.asarray(forward, dtype=np.float32)
+    np_, ns = _c.c_size_t(), _c.c_size_t()
+    lib = library()
+    args = (font, len(font), float(scale), _text_bytes(text), float(x), float(y),
+            float(maximum_width), int(align), int(baseline), fw.ctypes.data_as(_P),
+            1 if stroking else 0, float(line_width))
+    status = lib.cih_text_outline(*args, None, 0, None, 0, _c.byref(np_), _c.byref(ns))
+    if status not in (OK, ERROR_BUDGET):
+        _check(status, "cih_text_outline")
+    pts = np.zeros((max(np_.value, 1), 2), dtype=np.float32)
+    sub = np.zeros((max(ns.value, 1), 2), dtype=np.uint32)
+    _check(lib.cih_text_outline(*args, pts.ctypes.data_as(_P), pts.shape[0],
+                                sub.ctypes.data_as(_P), sub.shape[0], _c.byref(np_),
+                                _c.byref(ns)), "cih_text_outline")
+    return pts[:np_.value].copy(), sub[:ns.value].copy()
 
 
 def scene_seed(seed, index):

@@ -50,7 +50,7 @@ def build(verbose=False):
         _run([hipcc] + lang + common + (device if name.endswith(".hip") else []) +
              ["-c", os.path.join(CSRC, name), "-o", obj])
         objects.append(obj)
-    for name in ("cih_replay.cpp", "cih_scene.cpp"):
+    for name in ("cih_replay.cpp", "cih_scene.cpp", "cih_text.cpp"):
         obj = os.path.join(BUILD, name + ".o")
         _run(["g++", "-O2", "-fPIC", "-std=c++11", "-ffp-contract=off", "-fno-fast-math",
               "-fvisibility=hidden", "-fvisibility-inlines-hidden", "-Wall", "-Wextra",

@@ -228,6 +228,9 @@ __device__ line_path flatten(const frame &F, const draw_desc &D, bool stroking,
     const uint32_t *in_sub = (const uint32_t *)(F.payload + D.subpaths_at);
     uint32_t nsub = D.nsubpaths;
     float angular = angular_limit(stroking, D.line_width);
+    // a start point then 3 points per cubic, or (segments) 4 per cubic
+    const bool segments = (D.flags & CIH_DRAW_FLAG_SEGMENTS) != 0;
+    const uint32_t stride = segments ? 4u : 3u;
     // point and cubic prefixes over the subpaths
     uint64_t w = block_alloc(F, COUNTER_WORDS, 2 * (uint64_t)(nsub + 1), F.words_cap, slot);
     if (w == ~0ull)
@@ -237,7 +240,7 @@ __device__ line_path flatten(const frame &F, const draw_desc &D, bool stroking,
     for (uint32_t s = threadIdx.x; s < nsub; s += blockDim.x) {
         uint32_t c = in_sub[2 * s];
         sp_pts[s] = c;
-        sp_cub[s] = c >= 1 ? (c - 1) / 3 : 0;
+        sp_cub[s] = segments ? c / 4 : c >= 1 ? (c - 1) / 3 : 0;
     }
     __syncthreads();
     scan_array(sp_pts, nsub, lds);
@@ -248,7 +251,7 @@ __device__ line_path flatten(const frame &F, const draw_desc &D, bool stroking,
     uint32_t *cnt = F.words + cw;
     for (uint32_t g = threadIdx.x; g < cubics; g += blockDim.x) {
         uint32_t s = find_prefix(sp_cub, nsub, g);
-        uint32_t base = sp_pts[s] + 3 * (g - sp_cub[s]);
+        uint32_t base = sp_pts[s] + stride * (g - sp_cub[s]);
         count_points c;
         add_bezier(c, in[base], in[base + 1], in[base + 2], in[base + 3], angular);
         cnt[g] = c.points;
@@ -270,7 +273,7 @@ __device__ line_path flatten(const frame &F, const draw_desc &D, bool stroking,
     }
     for (uint32_t g = threadIdx.x; g < cubics; g += blockDim.x) {
         uint32_t s = find_prefix(sp_cub, nsub, g);
-        uint32_t base = sp_pts[s] + 3 * (g - sp_cub[s]);
+        uint32_t base = sp_pts[s] + stride * (g - sp_cub[s]);
         write_flat wf = {pts, cnt[g] + s + 1};
         add_bezier(wf, in[base], in[base + 1], in[base + 2], in[base + 3], angular);
     }

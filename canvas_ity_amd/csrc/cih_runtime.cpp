@@ -261,14 +261,20 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
     if (p.clip_mask >= kMaxMasks ||
         (p.kind == CIH_DRAW_CLIP && (p.image < 1 || p.image >= (int32_t)kMaxMasks)))
         return 0;
-    // subpath counts must add up to the points; bezier subpaths are 1 + 3k
+    // subpath counts must add up to the points; bezier subpaths are 1 + 3k,
+    // or 4k with explicit segment starts (fill and stroke paths only)
+    bool bezier = p.kind == CIH_DRAW_FILL_PATH || p.kind == CIH_DRAW_STROKE_PATH ||
+                  p.kind == CIH_DRAW_CLIP;
+    bool segments = (p.flags & CIH_DRAW_FLAG_SEGMENTS) != 0;
+    if ((p.flags & ~(CIH_DRAW_FLAG_EVENODD | CIH_DRAW_FLAG_SEGMENTS)) ||
+        (segments && (!bezier || p.kind == CIH_DRAW_CLIP)))
+        return 0;
     const unsigned char *sub = base + at + sizeof(p) + 8 * (size_t)p.points;
     uint64_t total = 0;
     for (uint32_t s = 0; s < p.subpaths; ++s) {
         uint32_t c;
         std::memcpy(&c, sub + 8 * (size_t)s, 4);
-        if (c == 0 || ((p.kind == CIH_DRAW_FILL_PATH || p.kind == CIH_DRAW_STROKE_PATH) &&
-                       (c - 1) % 3 != 0))
+        if (c == 0 || (bezier && (segments ? c % 4 != 0 : (c - 1) % 3 != 0)))
             return 0;
         total += c;
     }

@@ -20,7 +20,8 @@ OPS = dict(scale=1, rotate=2, translate=3, transform=4, set_transform=5,
            bezier_curve_to=27, arc_to=28, arc=29, rectangle=30, fill=31, stroke=32,
            clip=33, clear_rectangle=34, fill_rectangle=35, stroke_rectangle=36,
            draw_image=37, save=38, restore=39, define_image=40, put_image_data=41,
-           set_fill_rule=42)
+           set_fill_rule=42, set_font=43, fill_text=44, stroke_text=45, set_text_align=46,
+           set_text_baseline=47)
 
 # argument kinds: f float, i int
 KINDS = dict(scale="ff", rotate="f", translate="ff", transform="ffffff",
@@ -34,8 +35,11 @@ KINDS = dict(scale="ff", rotate="f", translate="ff", transform="ffffff",
              arc="fffffi", rectangle="ffff", fill="", stroke="", clip="",
              clear_rectangle="ffff", fill_rectangle="ffff", stroke_rectangle="ffff",
              draw_image="iffff", save="", restore="", put_image_data="iii",
-             set_fill_rule="i")
+             set_fill_rule="i", set_text_align="i", set_text_baseline="i")
 
+# text_align / text_baseline values (canvas_ity.hpp:150-156)
+ALIGN = dict(leftward=0, rightward=1, center=2, start=0, ending=1)
+BASELINE = dict(alphabetic=0, top=1, middle=2, bottom=3, hanging=4, ideographic=3)
 COMPOSITE = dict(source_in=1, source_copy=2, source_out=3, destination_in=4,
                  destination_atop=7, lighter=10, destination_over=11, destination_out=12,
                  source_atop=13, source_over=14, exclusive_or=15)
@@ -86,6 +90,33 @@ class Stream:
         self.words += np.frombuffer(rgba.tobytes(), dtype="<u4").tolist()
         return self
 
+    def _blob(self, data):
+        data = bytes(data)
+        padded = data + b"\0" * (-len(data) % 4)
+        self.words += np.frombuffer(padded, dtype="<u4").tolist()
+
+    def set_font(self, font, size):
+        """set_font(font, bytes, size); font None (or empty) only sets the size."""
+        font = bytes(font) if font else b""
+        self.words += [OPS["set_font"], len(font),
+                       struct.unpack("<I", struct.pack("<f", float(size)))[0]]
+        self._blob(font)
+        return self
+
+    def _text(self, name, text, x, y, maximum_width):
+        data = text.encode("utf-8") if isinstance(text, str) else bytes(text)
+        self.words += [OPS[name], len(data)]
+        for v in (x, y, maximum_width):
+            self.words.append(struct.unpack("<I", struct.pack("<f", float(v)))[0])
+        self._blob(data)
+        return self
+
+    def fill_text(self, text, x, y, maximum_width=1.0e30):
+        return self._text("fill_text", text, x, y, maximum_width)
+
+    def stroke_text(self, text, x, y, maximum_width=1.0e30):
+        return self._text("stroke_text", text, x, y, maximum_width)
+
     def tobytes(self):
         return np.asarray(self.words, dtype=np.uint32).tobytes()
 
@@ -93,8 +124,9 @@ class Stream:
 ARITY = {1: 2, 2: 1, 3: 2, 4: 6, 5: 6, 6: 1, 7: 1, 8: 4, 9: 1, 10: 2, 11: 1, 12: 1, 13: 1, 14: 1,
          16: 1, 17: 5, 18: 5, 19: 7, 20: 6, 21: 3, 22: 0, 23: 2, 24: 0, 25: 2, 26: 4, 27: 6,
          28: 5, 29: 6, 30: 4, 31: 0, 32: 0, 33: 0, 34: 4, 35: 4, 36: 4, 37: 5, 38: 0, 39: 0,
-         41: 3, 42: 1}
-DRAW_OPS = (31, 32, 34, 35, 36, 37)  # fill, stroke, clear/fill/stroke_rectangle, draw_image
+         41: 3, 42: 1, 46: 1, 47: 1}
+# fill, stroke, clear/fill/stroke_rectangle, draw_image, fill_text, stroke_text
+DRAW_OPS = (31, 32, 34, 35, 36, 37, 44, 45)
 
 
 def records(stream):
@@ -107,6 +139,8 @@ def records(stream):
             n = 1 + max(int(np.int32(words[at + 1])), 0)
         elif code == 40:
             n = 3 + int(words[at + 2]) * int(words[at + 3])
+        elif code in (43, 44, 45):
+            n = (2 if code == 43 else 4) + (int(words[at + 1]) + 3) // 4
         else:
             n = ARITY[code]
         out.append((code, at, n))

@@ -20,8 +20,9 @@
 //
 // clip() makes device-side masks that later draws are composited under;
 // is_point_in_path is a host-side query answered by the library without the
-// device.  Not provided (out of scope here): text (set_font, fill_text,
-// stroke_text, measure_text).  Extension: set_fill_rule.
+// device.  Text: the library lays glyph outlines out on the host
+// (cih_text_outline) and they are drawn as device-space paths like any fill
+// or stroke.  Extension: set_fill_rule.
 #ifndef CANVAS_ITY_HIP_HPP
 #define CANVAS_ITY_HIP_HPP
 
@@ -32,6 +33,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -131,8 +133,8 @@ public:
           shadow_offset_y(0.0f), line_cap(butt), line_join(miter),
           line_dash_offset(0.0f), text_align(start), text_baseline(alphabetic),
           size_x(width), size_y(height), global_alpha(1.0f), shadow_blur(0.0f),
-          line_width(1.0f), miter_limit(10.0f), rule(nonzero), clip_id(0), device(0),
-          handle(0), owned(true), saves(0) {
+          line_width(1.0f), miter_limit(10.0f), rule(nonzero), clip_id(0), face_units(0),
+          face_scale(0.0f), device(0), handle(0), owned(true), saves(0) {
         reset_state();
         if (width <= 0 || height <= 0)
             return; // state-only instance (save stack entries)
@@ -153,8 +155,8 @@ public:
           shadow_offset_y(0.0f), line_cap(butt), line_join(miter),
           line_dash_offset(0.0f), text_align(start), text_baseline(alphabetic),
           size_x(0), size_y(0), global_alpha(1.0f), shadow_blur(0.0f),
-          line_width(1.0f), miter_limit(10.0f), rule(nonzero), clip_id(0), device(0),
-          handle(borrowed), owned(false), saves(0) {
+          line_width(1.0f), miter_limit(10.0f), rule(nonzero), clip_id(0), face_units(0),
+          face_scale(0.0f), device(0), handle(borrowed), owned(false), saves(0) {
         reset_state();
         if (handle) {
             cih_canvas_size(handle, &size_x, &size_y);
@@ -417,8 +419,8 @@ public:
     }
 
     // ---- drawing (hpp:793-911, 3044-3203) ----
-    void fill() { emit_path(CIH_DRAW_FILL_PATH, fill_brush); }
-    void stroke() { emit_path(CIH_DRAW_STROKE_PATH, stroke_brush); }
+    void fill() { emit_path(CIH_DRAW_FILL_PATH, fill_brush, points, path_info(), 0); }
+    void stroke() { emit_path(CIH_DRAW_STROKE_PATH, stroke_brush, points, path_info(), 0); }
     void set_fill_rule(int new_rule) { rule = new_rule == evenodd ? evenodd : nonzero; }
     // hpp:828, 3057-3099: the current path's fill coverage, multiplied into
     // the current mask, becomes the new mask.  The device computes it (the
@@ -552,9 +554,46 @@ public:
         delete state;
     }
 
-    // Text layout fields kept for source compatibility (text is not drawn).
+    // ---- text (hpp:924-1029, 3205-3311) ----
     align_style text_align;
     baseline_style text_baseline;
+    // The font is copied (whole TrueType file); a null font or zero bytes
+    // keeps the current font and only sets the size.  Returns false, with no
+    // font left, if the file lacks what text needs.
+    bool set_font(unsigned char const *font, int bytes, float size) {
+        if (font && bytes) {
+            face.reset();
+            int units = 0;
+            if (bytes < 6 || cih_font_check(font, (size_t)bytes, &units) != CIH_OK)
+                return false;
+            face = std::make_shared<std::vector<unsigned char> const>(font, font + bytes);
+            face_units = units;
+        }
+        if (!face)
+            return false;
+        face_scale = size / static_cast<float>(face_units);
+        return true;
+    }
+    void fill_text(char const *text, float x, float y, float maximum_width = 1.0e30f) {
+        text_outline(text, x, y, maximum_width, false);
+        int saved_rule = rule; // text fills by the non-zero rule
+        rule = nonzero;
+        emit_path(CIH_DRAW_FILL_PATH, fill_brush, glyph_points, glyph_info,
+                  CIH_DRAW_FLAG_SEGMENTS);
+        rule = saved_rule;
+    }
+    void stroke_text(char const *text, float x, float y, float maximum_width = 1.0e30f) {
+        text_outline(text, x, y, maximum_width, true);
+        emit_path(CIH_DRAW_STROKE_PATH, stroke_brush, glyph_points, glyph_info,
+                  CIH_DRAW_FLAG_SEGMENTS);
+    }
+    float measure_text(char const *text) {
+        float width = 0.0f;
+        if (!face || !text ||
+            cih_text_measure(&(*face)[0], face->size(), face_scale, text, &width) != CIH_OK)
+            return 0.0f;
+        return width;
+    }
 
     // Extension: the device canvas behind this object (NULL if none).
     cih_canvas *device_canvas() { return handle; }
@@ -579,6 +618,11 @@ private:
     std::vector<detail::subpath> subpaths;
     int rule;
     uint32_t clip_id; // the current clip mask (0: the whole canvas)
+    std::shared_ptr<std::vector<unsigned char> const> face; // TrueType file
+    int face_units;   // its units per em
+    float face_scale; // font size / units per em
+    std::vector<detail::xy> glyph_points; // the last text's outline segments
+    std::vector<uint32_t> glyph_info;
     cih_device *device;
     cih_canvas *handle;
     bool owned;
@@ -618,6 +662,9 @@ private:
         to.fill_brush = from.fill_brush;
         to.stroke_brush = from.stroke_brush;
         to.rule = from.rule;
+        to.face = from.face;
+        to.face_units = from.face_units;
+        to.face_scale = from.face_scale;
     }
 
     void load_pattern(detail::brush &b, unsigned char const *image, int width, int height,
@@ -639,16 +686,47 @@ private:
 
     bool singular() const { return forward.a * forward.d - forward.b * forward.c == 0.0f; }
 
-    void emit_path(uint32_t kind, detail::brush const &b) {
-        if (singular())
-            return; // render_main draws nothing (hpp:2554) and stroke_lines bails (hpp:2072)
+    std::vector<uint32_t> path_info() const {
         std::vector<uint32_t> info(2 * subpaths.size());
         for (size_t k = 0; k < subpaths.size(); ++k) {
             info[2 * k] = (uint32_t)subpaths[k].count;
             info[2 * k + 1] = subpaths[k].closed ? 1u : 0u;
         }
-        encode(kind, b, b.type, points.empty() ? 0 : &points[0], points.size(),
-               info.empty() ? 0 : &info[0], subpaths.size(), inverse, forward);
+        return info;
+    }
+
+    void emit_path(uint32_t kind, detail::brush const &b, std::vector<detail::xy> const &pts,
+                   std::vector<uint32_t> const &info, uint32_t flags) {
+        if (singular())
+            return; // render_main draws nothing (hpp:2554) and stroke_lines bails (hpp:2072)
+        encode(kind, b, b.type, pts.empty() ? 0 : &pts[0], pts.size(),
+               info.empty() ? 0 : &info[0], info.size() / 2, inverse, forward, 0, flags);
+    }
+
+    // text_to_lines (hpp:1793-1846): the outline of `text` as segment-start
+    // cubics in glyph_points / glyph_info; empty without a font, text or
+    // positive maximum width (the draw still composites, as render_main does).
+    void text_outline(char const *text, float x, float y, float maximum_width, bool stroking) {
+        glyph_points.clear();
+        glyph_info.clear();
+        if (!face || !text || maximum_width <= 0.0f)
+            return;
+        const float fw[6] = {forward.a, forward.b, forward.c, forward.d, forward.e, forward.f};
+        size_t np = 0, ns = 0;
+        int status = CIH_ERROR_BUDGET;
+        for (int attempt = 0; attempt < 2 && status == CIH_ERROR_BUDGET; ++attempt) {
+            glyph_points.resize(std::max(np, glyph_points.capacity()));
+            glyph_info.resize(std::max(2 * ns, glyph_info.capacity()));
+            status = cih_text_outline(
+                &(*face)[0], face->size(), face_scale, text, x, y, maximum_width,
+                (int)text_align, (int)text_baseline, fw, stroking ? 1 : 0, line_width,
+                glyph_points.empty() ? 0 : &glyph_points[0].x, glyph_points.size(),
+                glyph_info.empty() ? 0 : &glyph_info[0], glyph_info.size() / 2, &np, &ns);
+        }
+        if (status != CIH_OK)
+            np = ns = 0;
+        glyph_points.resize(np);
+        glyph_info.resize(2 * ns);
     }
 
     void emit_lines(uint32_t kind, detail::brush const &b, detail::xy const *pts, size_t n,
@@ -663,7 +741,7 @@ private:
     void encode(uint32_t kind, detail::brush const &b, int brush_kind, detail::xy const *pts,
                 size_t npts, uint32_t const *info, size_t nsub,
                 detail::affine const &paint_inverse, detail::affine const &fwd,
-                int32_t new_mask = 0) {
+                int32_t new_mask = 0, uint32_t flags = 0) {
         if (!handle)
             return;
         cih_draw_packet p;
@@ -671,7 +749,8 @@ private:
         p.kind = kind;
         p.clip_mask = clip_id;
         p.op = (uint32_t)global_composite_operation;
-        p.flags = rule == evenodd && (kind == CIH_DRAW_FILL_PATH) ? CIH_DRAW_FLAG_EVENODD : 0u;
+        p.flags = flags | (rule == evenodd && kind == CIH_DRAW_FILL_PATH ? CIH_DRAW_FLAG_EVENODD
+                                                                          : 0u);
         p.global_alpha = global_alpha;
         p.line_width = line_width;
         p.miter_limit = miter_limit;

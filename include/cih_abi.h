@@ -77,6 +77,13 @@ enum cih_brush_kind {
 };
 
 #define CIH_DRAW_FLAG_EVENODD 1u /* extension: even-odd fill rule           */
+/* Path packets (fill / stroke) whose cubics carry their own start points:
+ * 4 points per cubic (start, control 1, control 2, end), subpaths of 4k
+ * points whose polyline begins at the first cubic's start.  Glyph outlines
+ * use it: the reference starts each glyph curve at the last point its
+ * tessellation emitted (canvas_ity.hpp:1660-1665, 1674-1686), which can
+ * differ by rounding from the previous curve's end point. */
+#define CIH_DRAW_FLAG_SEGMENTS 2u
 
 typedef struct cih_draw_packet {
     uint32_t bytes;       /* padded size of header + arrays                   */
@@ -203,6 +210,31 @@ CIH_API int cih_canvas_replay(cih_canvas *canvas, const void *stream, size_t byt
  * float arithmetic.  A host-side query: it needs no device. */
 CIH_API int cih_path_contains(const float *points, size_t npoints, const uint32_t *subpaths,
                               size_t nsubpaths, float x, float y, int *inside);
+
+/* ---- text: TrueType layout, on the host ------------------------------------
+ * Host-side functions (no device) behind canvas_ity_hip.hpp's set_font,
+ * measure_text, fill_text and stroke_text; `font` is a whole TrueType file
+ * and `scale` the font size over its units per em (hpp:3271-3273).
+ * cih_font_check replaces set_font's table checks (canvas_ity.hpp:955,
+ * 3205-3274): CIH_OK with the head table's units per em, or
+ * CIH_ERROR_MALFORMED.  cih_text_measure replaces measure_text (hpp:1025,
+ * 3297-3311).  cih_text_outline replaces text_to_lines and add_glyph
+ * (hpp:1533-1846): the glyph outlines of `text` placed at (x, y) under the
+ * affine `forward` with text_align `align`, text_baseline `baseline` and
+ * `maximum_width`, as the geometry of a CIH_DRAW_FLAG_SEGMENTS packet in
+ * device space, one closed subpath per glyph contour; `stroking` and
+ * `line_width` choose the flattening the segment start points follow
+ * (hpp:1497-1500).  If an array is too small it returns CIH_ERROR_BUDGET
+ * with *npoints and *nsubpaths set to the sizes needed. */
+CIH_API int cih_font_check(const unsigned char *font, size_t bytes, int *units_per_em);
+CIH_API int cih_text_measure(const unsigned char *font, size_t font_bytes, float scale,
+                             const char *text, float *width);
+CIH_API int cih_text_outline(const unsigned char *font, size_t font_bytes, float scale,
+                             const char *text, float x, float y, float maximum_width,
+                             int align, int baseline, const float forward[6], int stroking,
+                             float line_width, float *points, size_t point_capacity,
+                             uint32_t *subpaths, size_t subpath_capacity, size_t *npoints,
+                             size_t *nsubpaths);
 
 /* ---- batches: many independent canvases in shared frames ------------------
  * A batch replays every command stream through canvas_ity_hip.hpp (host

@@ -11,6 +11,7 @@
 #include "cih_stream.h"
 
 #include <cstddef>
+#include <string>
 #include <vector>
 
 namespace cih {
@@ -28,12 +29,20 @@ struct replayer {
     Canvas &canvas;
     std::vector<stream_image> images;
     std::vector<float> dash;
+    std::string string;
 
     explicit replayer(Canvas &target)
         : canvas(target), images(CIH_STREAM_MAX_IMAGES) {}
 
     static float f(const uint32_t *args, int index) { return cih_w2f(args[index]); }
     static int i(const uint32_t *args, int index) { return (int)(int32_t)args[index]; }
+
+    // A text record's bytes as a C string (they end at a NUL, if any).
+    const char *text(const uint32_t *args) {
+        const char *bytes = reinterpret_cast<const char *>(args + 4);
+        string.assign(bytes, bytes + i(args, 0));
+        return string.c_str();
+    }
 
     const stream_image *image(int id) const {
         if (id < 0 || id >= CIH_STREAM_MAX_IMAGES || !images[(size_t)id].texels)
@@ -173,6 +182,24 @@ struct replayer {
             break;
         }
         case CIH_OP_SET_FILL_RULE: set_fill_rule(canvas, i(a, 0), 0); break;
+        case CIH_OP_SET_FONT: {
+            int bytes = i(a, 0);
+            const unsigned char *font = reinterpret_cast<const unsigned char *>(a + 2);
+            canvas.set_font(bytes ? font : 0, bytes, f(a, 1));
+            break;
+        }
+        case CIH_OP_FILL_TEXT:
+            canvas.fill_text(text(a), f(a, 1), f(a, 2), f(a, 3));
+            break;
+        case CIH_OP_STROKE_TEXT:
+            canvas.stroke_text(text(a), f(a, 1), f(a, 2), f(a, 3));
+            break;
+        case CIH_OP_SET_TEXT_ALIGN:
+            canvas.text_align = static_cast<decltype(canvas.text_align)>(i(a, 0));
+            break;
+        case CIH_OP_SET_TEXT_BASELINE:
+            canvas.text_baseline = static_cast<decltype(canvas.text_baseline)>(i(a, 0));
+            break;
         default: return -1;
         }
         return 0;

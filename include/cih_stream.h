@@ -16,7 +16,12 @@
  * (canvas_ity.hpp:194-1148); the argument order is the method's.  Images are
  * defined once with CIH_OP_DEFINE_IMAGE (id, width, height, then width*height
  * RGBA8 texels packed four bytes per word) and referenced by id from
- * CIH_OP_SET_PATTERN, CIH_OP_DRAW_IMAGE and CIH_OP_PUT_IMAGE_DATA.
+ * CIH_OP_SET_PATTERN, CIH_OP_DRAW_IMAGE and CIH_OP_PUT_IMAGE_DATA.  Font
+ * and text records carry their byte count first and their bytes last, packed
+ * four to a word in memory order and zero-padded to a whole word: a font's
+ * TrueType file (count 0 with set_font's null font, which only sets the size)
+ * and a text's UTF-8 bytes without the terminating NUL (the text ends at its
+ * first NUL byte, as a C string does).
  * CIH_OP_SET_FILL_RULE is an extension (the reference only fills non-zero).
  */
 #ifndef CIH_STREAM_H
@@ -33,6 +38,7 @@ extern "C" {
 #define CIH_STREAM_MAGIC 0x53484943u /* "CIHS" */
 #define CIH_STREAM_VERSION 1u
 #define CIH_STREAM_MAX_IMAGES 4096
+#define CIH_STREAM_MAX_BLOB (64 << 20) /* font and text records, bytes */
 
 enum cih_opcode {
     CIH_OP_SCALE = 1,              /* f x, f y                     hpp:216  */
@@ -77,6 +83,11 @@ enum cih_opcode {
     CIH_OP_DEFINE_IMAGE = 40,      /* i id, i w, i h, texels       --       */
     CIH_OP_PUT_IMAGE_DATA = 41,    /* i image, i x, i y            hpp:1123 */
     CIH_OP_SET_FILL_RULE = 42,     /* i rule (0 nonzero, 1 evenodd) ext.    */
+    CIH_OP_SET_FONT = 43,          /* i bytes, f size, font bytes  hpp:955  */
+    CIH_OP_FILL_TEXT = 44,         /* i bytes, f x,y,max, text     hpp:980  */
+    CIH_OP_STROKE_TEXT = 45,       /* i bytes, f x,y,max, text     hpp:1006 */
+    CIH_OP_SET_TEXT_ALIGN = 46,    /* i align (field assignment)   hpp:924  */
+    CIH_OP_SET_TEXT_BASELINE = 47, /* i baseline (field)           hpp:937  */
     CIH_OP_COUNT_
 };
 
@@ -87,7 +98,8 @@ static inline int cih_op_arity(uint32_t op)
         0,  2, 1, 2, 6, 6, 1, 1, 4, 1, 2,   /*  0..10 */
         1,  1, 1, 1, -1, 1, 5, 5, 7, 6, 3,  /* 11..21 */
         0,  2, 0, 2, 4, 6, 5, 6, 4, 0, 0,   /* 22..32 */
-        0,  4, 4, 4, 5, 0, 0, -1, 3, 1 };   /* 33..42 */
+        0,  4, 4, 4, 5, 0, 0, -1, 3, 1,     /* 33..42 */
+        -1, -1, -1, 1, 1 };                 /* 43..47 */
     return op > 0 && op < CIH_OP_COUNT_ ? arity[op] : -2;
 }
 
@@ -159,6 +171,16 @@ static inline int cih_reader_next(cih_reader *reader, cih_record *record)
         if (n < -1)
             return -1;
         nargs = 1 + (size_t)(n > 0 ? n : 0);
+    } else if (op == CIH_OP_SET_FONT || op == CIH_OP_FILL_TEXT ||
+               op == CIH_OP_STROKE_TEXT) {
+        /* a byte count, then fixed words, then the bytes packed four per word */
+        size_t fixed = op == CIH_OP_SET_FONT ? 2 : 4;
+        if (left < fixed)
+            return -1;
+        int32_t n = (int32_t)args[0];
+        if (n < 0 || n > CIH_STREAM_MAX_BLOB)
+            return -1;
+        nargs = fixed + ((size_t)n + 3) / 4;
     } else { /* CIH_OP_DEFINE_IMAGE */
         if (left < 3)
             return -1;

@@ -67,9 +67,36 @@ def _load(kind):
         fn = lib.cih_ref_is_point_in_path
         fn.restype = ctypes.c_int
         fn.argtypes = [_P, ctypes.c_float, ctypes.c_float]
+        fn = lib.cih_ref_set_font
+        fn.restype = ctypes.c_int
+        fn.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_float, ctypes.c_char_p,
+                       ctypes.POINTER(ctypes.c_float)]
+    else:
+        fn = lib.cih_oracle_measure_text
+        fn.restype = ctypes.c_float
+        fn.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_char_p]
     fns["lib"] = lib
     _libs[kind] = fns
     return fns
+
+
+def measure_text(font, size, text, kind="port"):
+    """measure_text of `text` (str or UTF-8 bytes) after set_font(font, size)
+    on a fresh canvas (0.0 if the font is rejected)."""
+    data = text.encode("utf-8") if isinstance(text, str) else bytes(text)
+    fns = _load(kind)
+    if kind == "port":
+        return fns["lib"].cih_oracle_measure_text(font, len(font), size, data)
+    width = ctypes.c_float()
+    fns["lib"].cih_ref_set_font(font, len(font), size, data, ctypes.byref(width))
+    return width.value
+
+
+def font_accepted(font, kind="reference"):
+    """Whether set_font accepts a font (the reference build's answer)."""
+    width = ctypes.c_float()
+    return _load(kind)["lib"].cih_ref_set_font(font, len(font), 16.0, b"",
+                                               ctypes.byref(width)) == 1
 
 
 def available(kind):

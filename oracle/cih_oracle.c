@@ -15,7 +15,13 @@
  * aligned and fractional rectangles, polygon areas, blur weights).  Where the
  * reference build is unavailable the closed-form and KAT checks still run.
  *
- * Out of scope: text (hpp:1526-1846, 3205-3311) and is_point_in_path.
+ * Text (hpp:1299-1314, 1526-1846, 3205-3311) is restated too: glyph outlines
+ * go straight into the polylines, as in the reference.  Font reads are
+ * bounds-checked (out-of-range bytes read as 0), which only matters for
+ * malformed fonts, where the reference's reads are undefined.
+ *
+ * Out of scope: is_point_in_path (answered by the library on the host and
+ * checked against the reference directly).
  */
 #include "cih_oracle.h"
 #include "cih_stream.h"
@@ -89,6 +95,13 @@ struct cih_oracle { /* canvas state, hpp:1150-1172 */
     rgba *bitmap;
     struct cih_oracle *saves;
     int evenodd; /* extension: even-odd fill rule */
+    int text_align, text_baseline;
+    struct { /* font_face, hpp:166-168: offset table + kept tables */
+        unsigned char *data;
+        size_t n;
+        int cmap, glyf, head, hhea, hmtx, loca, maxp, os_2;
+        float scale;
+    } face;
 };
 
 enum { OP_SOURCE_OVER = 14, OP_DESTINATION_OUT = 12 };
@@ -1225,6 +1238,386 @@ static void clip(cih_oracle *c) /* hpp:3057-3099 */
     }
 }
 
+/* ---- text (hpp:1299-1314, 1526-1846, 3205-3311) --------------------------- */
+
+static int font_u8(const cih_oracle *c, long at) /* hpp:1300-1301 */
+{
+    return at >= 0 && (size_t)at < c->face.n ? c->face.data[at] : 0;
+}
+static int font_s8(const cih_oracle *c, long at) { return (int)(signed char)font_u8(c, at); }
+static int font_u16(const cih_oracle *c, long at) /* hpp:1305-1307 */
+{
+    return font_u8(c, at) << 8 | font_u8(c, at + 1);
+}
+static int font_s16(const cih_oracle *c, long at) { return (int)(short)font_u16(c, at); }
+static int font_s32(const cih_oracle *c, long at) /* hpp:1311-1314 */
+{
+    return (int)((unsigned)font_u8(c, at) << 24 | (unsigned)font_u8(c, at + 1) << 16 |
+                 (unsigned)font_u8(c, at + 2) << 8 | (unsigned)font_u8(c, at + 3));
+}
+
+static void face_clear(cih_oracle *c)
+{
+    free(c->face.data);
+    c->face.data = NULL;
+    c->face.n = 0;
+    c->face.cmap = c->face.glyf = c->face.head = c->face.hhea = 0;
+    c->face.hmtx = c->face.loca = c->face.maxp = c->face.os_2 = 0;
+}
+
+static void face_append(cih_oracle *c, const unsigned char *bytes, size_t n)
+{
+    c->face.data = (unsigned char *)realloc(c->face.data, c->face.n + n + 1);
+    if (!c->face.data)
+        abort();
+    memcpy(c->face.data + c->face.n, bytes, n);
+    c->face.n += n;
+}
+
+/* set_font (hpp:3205-3274): keeps the offset table, then appends the eight
+ * tables text uses, recording where each landed. */
+static int set_font(cih_oracle *c, const unsigned char *font, int bytes, float size)
+{
+    if (font && bytes) {
+        int version, tables, index;
+        face_clear(c);
+        if (bytes < 6)
+            return 0;
+        version = (int)((unsigned)font[0] << 24 | (unsigned)font[1] << 16 |
+                        (unsigned)font[2] << 8 | (unsigned)font[3]);
+        tables = font[4] << 8 | font[5];
+        if ((version != 0x00010000 && version != 0x74727565) || bytes < tables * 16 + 12)
+            return 0;
+        face_append(c, font, (size_t)(tables * 16 + 12));
+        for (index = 0; index < tables; ++index) {
+            int tag = font_s32(c, index * 16 + 12);
+            long offset = font_s32(c, index * 16 + 20);
+            long span = font_s32(c, index * 16 + 24);
+            int place = (int)c->face.n;
+            if (offset < 0 || span < 0 || bytes < offset + span) {
+                face_clear(c);
+                return 0;
+            }
+            if (tag == 0x636d6170) c->face.cmap = place;
+            else if (tag == 0x676c7966) c->face.glyf = place;
+            else if (tag == 0x68656164) c->face.head = place;
+            else if (tag == 0x68686561) c->face.hhea = place;
+            else if (tag == 0x686d7478) c->face.hmtx = place;
+            else if (tag == 0x6c6f6361) c->face.loca = place;
+            else if (tag == 0x6d617870) c->face.maxp = place;
+            else if (tag == 0x4f532f32) c->face.os_2 = place;
+            else continue;
+            face_append(c, font + offset, (size_t)span);
+        }
+        if (!c->face.cmap || !c->face.glyf || !c->face.head || !c->face.hhea ||
+            !c->face.hmtx || !c->face.loca || !c->face.maxp || !c->face.os_2) {
+            face_clear(c);
+            return 0;
+        }
+    }
+    if (!c->face.n)
+        return 0;
+    c->face.scale = size / (float)font_u16(c, c->face.head + 18);
+    return 1;
+}
+
+/* character_to_glyph (hpp:1709-1784). */
+static int character_to_glyph(const cih_oracle *c, const unsigned char *text, long *index)
+{
+    unsigned lead = text[*index];
+    int bytes = (lead & 0x80) == 0x00 ? 1 : (lead & 0xe0) == 0xc0 ? 2
+              : (lead & 0xf0) == 0xe0 ? 3 : (lead & 0xf8) == 0xf0 ? 4 : 0;
+    static const int masks[] = {0x0, 0x7f, 0x1f, 0x0f, 0x07};
+    int codepoint = bytes ? (int)(lead & (unsigned)masks[bytes]) : 0xfffd;
+    int tables, table;
+    long format_12 = 0, format_4 = 0, format_0 = 0;
+    ++*index;
+    while (--bytes > 0) {
+        if ((text[*index] & 0xc0) == 0x80)
+            codepoint = codepoint << 6 | (text[(*index)++] & 0x3f);
+        else {
+            codepoint = 0xfffd;
+            break;
+        }
+    }
+    if (codepoint == '\t' || codepoint == '\v' || codepoint == '\f' || codepoint == '\r' ||
+        codepoint == '\n')
+        codepoint = ' ';
+    tables = font_u16(c, c->face.cmap + 2);
+    for (table = 0; table < tables; ++table) {
+        int platform = font_u16(c, c->face.cmap + table * 8 + 4);
+        int encoding = font_u16(c, c->face.cmap + table * 8 + 6);
+        long offset = font_s32(c, c->face.cmap + table * 8 + 8);
+        int format = font_u16(c, c->face.cmap + offset);
+        if (platform == 3 && encoding == 10 && format == 12)
+            format_12 = c->face.cmap + offset;
+        else if (platform == 3 && encoding == 1 && format == 4)
+            format_4 = c->face.cmap + offset;
+        else if (format == 0)
+            format_0 = c->face.cmap + offset;
+    }
+    if (format_12) {
+        long groups = font_s32(c, format_12 + 12), group;
+        for (group = 0; group < groups && (size_t)(format_12 + 16 + group * 12) < c->face.n;
+             ++group) {
+            int start = font_s32(c, format_12 + 16 + group * 12);
+            int end = font_s32(c, format_12 + 20 + group * 12);
+            int glyph = font_s32(c, format_12 + 24 + group * 12);
+            if (start <= codepoint && codepoint <= end)
+                return codepoint - start + glyph;
+        }
+    } else if (format_4) {
+        int segments = font_u16(c, format_4 + 6), segment;
+        long end_array = format_4 + 14;
+        long start_array = end_array + 2 + segments;
+        long delta_array = start_array + segments;
+        long range_array = delta_array + segments;
+        for (segment = 0; segment < segments; segment += 2) {
+            int start = font_u16(c, start_array + segment);
+            int end = font_u16(c, end_array + segment);
+            int delta = font_s16(c, delta_array + segment);
+            int range = font_u16(c, range_array + segment);
+            if (start <= codepoint && codepoint <= end)
+                return range ? font_u16(c, range_array + segment +
+                                               (long)(codepoint - start) * 2 + range)
+                             : (codepoint + delta) & 0xffff;
+        }
+    } else if (format_0 && 0 <= codepoint && codepoint < 256)
+        return font_u8(c, format_0 + 6 + codepoint);
+    return 0;
+}
+
+static float measure_text(const cih_oracle *c, const char *text) /* hpp:3297-3311 */
+{
+    int hmetrics, width = 0;
+    long index = 0;
+    if (!c->face.n || !text)
+        return 0.0f;
+    hmetrics = font_u16(c, c->face.hhea + 34);
+    while (text[index]) {
+        int glyph = character_to_glyph(c, (const unsigned char *)text, &index);
+        int entry = glyph < hmetrics - 1 ? glyph : hmetrics - 1; /* std::min */
+        width += font_u16(c, c->face.hmtx + (long)entry * 4);
+    }
+    return (float)width * c->face.scale;
+}
+
+/* add_glyph (hpp:1533-1696): the glyph's contours, through c->forward, as
+ * closed polylines appended to c->lines. */
+static void add_glyph(cih_oracle *c, int glyph, float angular, int depth)
+{
+    int loc_format = font_u16(c, c->face.head + 50);
+    long offset = c->face.glyf + (loc_format ? (long)font_s32(c, c->face.loca + (long)glyph * 4)
+                                             : (long)font_u16(c, c->face.loca + (long)glyph * 2) * 2);
+    long next = c->face.glyf + (loc_format ? (long)font_s32(c, c->face.loca + (long)glyph * 4 + 4)
+                                           : (long)font_u16(c, c->face.loca + (long)glyph * 2 + 2) * 2);
+    int contours, hmetrics, left_side_bearing, x_min, points, instructions;
+    long flags_array, flags_size = 0, x_size = 0, x_array, y_array, index;
+    int x, y, flags = 0, repeated = 0, contour;
+    if (offset == next)
+        return;
+    contours = font_s16(c, offset);
+    if (contours < 0) { /* composite: transformed components */
+        if (depth >= 16)
+            return;
+        offset += 10;
+        for (;;) {
+            int fl = font_u16(c, offset);
+            int component = font_u16(c, offset + 2);
+            float e, f, a, b, cc, d;
+            affine saved_forward, saved_inverse;
+            if (!(fl & 2))
+                return;
+            e = (float)(fl & 1 ? font_s16(c, offset + 4) : font_s8(c, offset + 4));
+            f = (float)(fl & 1 ? font_s16(c, offset + 6) : font_s8(c, offset + 5));
+            offset += fl & 1 ? 8 : 6;
+            a = fl & 200 ? (float)font_s16(c, offset) / 16384.0f : 1.0f;
+            b = fl & 128 ? (float)font_s16(c, offset + 2) / 16384.0f : 0.0f;
+            cc = fl & 128 ? (float)font_s16(c, offset + 4) / 16384.0f : 0.0f;
+            d = fl & 8 ? a
+              : fl & 64 ? (float)font_s16(c, offset + 2) / 16384.0f
+              : fl & 128 ? (float)font_s16(c, offset + 6) / 16384.0f
+              : 1.0f;
+            offset += fl & 8 ? 2 : fl & 64 ? 4 : fl & 128 ? 8 : 0;
+            saved_forward = c->forward;
+            saved_inverse = c->inverse;
+            transform(c, a, b, cc, d, e, f);
+            add_glyph(c, component, angular, depth + 1);
+            c->forward = saved_forward;
+            c->inverse = saved_inverse;
+            if (!(fl & 32) || (size_t)offset >= c->face.n)
+                return;
+        }
+    }
+    hmetrics = font_u16(c, c->face.hhea + 34);
+    left_side_bearing = glyph < hmetrics
+        ? font_s16(c, c->face.hmtx + (long)glyph * 4 + 2)
+        : font_s16(c, c->face.hmtx + (long)hmetrics * 2 + (long)glyph * 2);
+    x_min = font_s16(c, offset + 2);
+    points = font_u16(c, offset + 8 + (long)contours * 2) + 1;
+    instructions = font_u16(c, offset + 10 + (long)contours * 2);
+    flags_array = offset + 12 + (long)contours * 2 + instructions;
+    for (index = 0; index < points;) {
+        int fl, rep;
+        if ((size_t)(flags_array + flags_size) >= c->face.n)
+            return;
+        fl = font_u8(c, flags_array + flags_size++);
+        rep = fl & 8 ? font_u8(c, flags_array + flags_size++) + 1 : 1;
+        x_size += rep * (fl & 2 ? 1 : fl & 16 ? 0 : 2);
+        index += rep;
+    }
+    x_array = flags_array + flags_size;
+    y_array = x_array + x_size;
+    x = left_side_bearing - x_min;
+    y = 0;
+    index = 0;
+    for (contour = 0; contour < contours; ++contour) {
+        long beginning = index;
+        long ending = font_u16(c, offset + 10 + (long)contour * 2);
+        xy begin_point = mkxy(0.0f, 0.0f), end_point = mkxy(0.0f, 0.0f);
+        int begin_on = 0, end_on = 0;
+        size_t first = c->lines.points.n;
+        subpath entry;
+        for (; index <= ending && index < points; ++index) {
+            xy point;
+            int on_curve;
+            if (repeated)
+                --repeated;
+            else {
+                flags = font_u8(c, flags_array++);
+                if (flags & 8)
+                    repeated = font_u8(c, flags_array++);
+            }
+            if (flags & 2)
+                x += font_u8(c, x_array) * (flags & 16 ? 1 : -1);
+            else if (!(flags & 16))
+                x += font_s16(c, x_array);
+            if (flags & 4)
+                y += font_u8(c, y_array) * (flags & 32 ? 1 : -1);
+            else if (!(flags & 32))
+                y += font_s16(c, y_array);
+            x_array += flags & 2 ? 1 : flags & 16 ? 0 : 2;
+            y_array += flags & 4 ? 1 : flags & 32 ? 0 : 2;
+            point = apply(&c->forward, mkxy((float)x, (float)y));
+            on_curve = flags & 1;
+            if (index == beginning) {
+                begin_point = point;
+                begin_on = on_curve;
+                if (on_curve)
+                    PUSH(c->lines.points, point);
+            } else {
+                xy point_2 = on_curve ? point : lerp(end_point, point, 0.5f);
+                if (c->lines.points.n == first || (end_on && on_curve))
+                    PUSH(c->lines.points, point_2);
+                else if (!end_on || on_curve) {
+                    xy point_1 = c->lines.points.v[c->lines.points.n - 1];
+                    xy control_1 = lerp(point_1, end_point, 2.0f / 3.0f);
+                    xy control_2 = lerp(point_2, end_point, 2.0f / 3.0f);
+                    add_bezier(c, point_1, control_1, control_2, point_2, angular);
+                }
+            }
+            end_point = point;
+            end_on = on_curve;
+        }
+        if (c->lines.points.n == first)
+            continue; /* one off-curve point: no outline */
+        if (begin_on ^ end_on) {
+            xy point_1 = c->lines.points.v[c->lines.points.n - 1];
+            xy point_2 = c->lines.points.v[first];
+            xy control = end_on ? begin_point : end_point;
+            xy control_1 = lerp(point_1, control, 2.0f / 3.0f);
+            xy control_2 = lerp(point_2, control, 2.0f / 3.0f);
+            add_bezier(c, point_1, control_1, control_2, point_2, angular);
+        } else if (!begin_on && !end_on) {
+            xy point_1 = c->lines.points.v[c->lines.points.n - 1];
+            xy split = lerp(begin_point, end_point, 0.5f);
+            xy point_2 = c->lines.points.v[first];
+            xy left_1 = lerp(point_1, end_point, 2.0f / 3.0f);
+            xy left_2 = lerp(split, end_point, 2.0f / 3.0f);
+            xy right_1 = lerp(split, begin_point, 2.0f / 3.0f);
+            xy right_2 = lerp(point_2, begin_point, 2.0f / 3.0f);
+            add_bezier(c, point_1, left_1, left_2, split, angular);
+            add_bezier(c, split, right_1, right_2, point_2, angular);
+        }
+        PUSH(c->lines.points, c->lines.points.v[first]);
+        entry.count = c->lines.points.n - first;
+        entry.closed = 1;
+        PUSH(c->lines.subpaths, entry);
+    }
+}
+
+/* text_to_lines (hpp:1793-1846). */
+static void text_to_lines(cih_oracle *c, const char *text, xy position, float maximum_width,
+                          int stroking)
+{
+    const float tolerance = 0.125f;
+    float ratio = tolerance / max_std(0.5f * c->line_width, tolerance);
+    float angular = stroking ? (ratio - 2.0f) * ratio * 2.0f + 1.0f : -1.0f;
+    float width, reduction, units_per_em, ascender, descender, normalize;
+    xy scaling;
+    affine saved_forward, saved_inverse;
+    int hmetrics, place = 0;
+    long index = 0;
+    c->lines.points.n = 0;
+    c->lines.subpaths.n = 0;
+    if (!c->face.n || !text || maximum_width <= 0.0f)
+        return;
+    width = maximum_width == 1.0e30f && c->text_align == 0 ? 0.0f : measure_text(c, text);
+    reduction = maximum_width / max_std(maximum_width, width);
+    if (c->text_align == 1) /* rightward */
+        position.x -= width * reduction;
+    else if (c->text_align == 2) /* center */
+        position.x -= 0.5f * width * reduction;
+    scaling = scale(c->face.scale, mkxy(reduction, 1.0f));
+    units_per_em = (float)font_u16(c, c->face.head + 18);
+    ascender = (float)font_s16(c, c->face.os_2 + 68);
+    descender = (float)font_s16(c, c->face.os_2 + 70);
+    normalize = c->face.scale * units_per_em / (ascender - descender);
+    if (c->text_baseline == 1) /* top */
+        position.y += ascender * normalize;
+    else if (c->text_baseline == 2) /* middle */
+        position.y += (ascender + descender) * 0.5f * normalize;
+    else if (c->text_baseline == 3) /* bottom, ideographic */
+        position.y += descender * normalize;
+    else if (c->text_baseline == 4) /* hanging */
+        position.y += 0.6f * c->face.scale * units_per_em;
+    saved_forward = c->forward;
+    saved_inverse = c->inverse;
+    hmetrics = font_u16(c, c->face.hhea + 34);
+    while (text[index]) {
+        int glyph = character_to_glyph(c, (const unsigned char *)text, &index);
+        int entry = glyph < hmetrics - 1 ? glyph : hmetrics - 1;
+        c->forward = saved_forward;
+        transform(c, scaling.x, 0.0f, 0.0f, -scaling.y,
+                  position.x + (float)place * scaling.x, position.y);
+        add_glyph(c, glyph, angular, 0);
+        place += font_u16(c, c->face.hmtx + (long)entry * 4);
+    }
+    c->forward = saved_forward;
+    c->inverse = saved_inverse;
+}
+
+static void fill_text(cih_oracle *c, const char *text, float x, float y,
+                      float maximum_width) /* hpp:3276-3284 */
+{
+    int rule = c->evenodd;
+    text_to_lines(c, text, mkxy(x, y), maximum_width, 0);
+    c->evenodd = 0;
+    render_main(c, &c->fill_brush, 0);
+    c->evenodd = rule;
+}
+
+static void stroke_text(cih_oracle *c, const char *text, float x, float y,
+                        float maximum_width) /* hpp:3286-3295 */
+{
+    int rule = c->evenodd;
+    text_to_lines(c, text, mkxy(x, y), maximum_width, 1);
+    stroke_lines(c);
+    c->evenodd = 0;
+    render_main(c, &c->stroke_brush, 0);
+    c->evenodd = rule;
+}
+
 static void fill_rectangle(cih_oracle *c, float x, float y, float w, float h) /* hpp:3155-3172 */
 {
     subpath s;
@@ -1381,6 +1774,7 @@ static void free_state(cih_oracle *c)
     free(c->runs.v);
     free(c->mask.v);
     free(c->bitmap);
+    free(c->face.data);
 }
 
 void cih_oracle_destroy(cih_oracle *c)
@@ -1425,6 +1819,17 @@ static void copy_saved(cih_oracle *to, const cih_oracle *from) /* hpp:3410-3465 
         memcpy(to->mask.v, from->mask.v, from->mask.n * sizeof(pixel_run));
     to->mask.n = from->mask.n;
     to->evenodd = from->evenodd;
+    to->text_align = from->text_align;
+    to->text_baseline = from->text_baseline;
+    free(to->face.data);
+    to->face = from->face;
+    to->face.data = NULL;
+    if (from->face.n) {
+        to->face.data = (unsigned char *)malloc(from->face.n);
+        if (!to->face.data)
+            abort();
+        memcpy(to->face.data, from->face.data, from->face.n);
+    }
 }
 
 static void save(cih_oracle *c)
@@ -1494,6 +1899,18 @@ void cih_oracle_put_image_data(cih_oracle *c, const unsigned char *image, int wi
                            image[index + 2] / 255.0f, image[index + 3] / 255.0f);
             c->bitmap[cy * c->size_x + cx] = premultiplied(linearized(color));
         }
+}
+
+float cih_oracle_measure_text(const unsigned char *font, size_t bytes, float size,
+                              const char *text)
+{
+    cih_oracle state;
+    float width;
+    init_state(&state, 0, 0);
+    set_font(&state, font, (int)bytes, size);
+    width = measure_text(&state, text);
+    free_state(&state);
+    return width;
 }
 
 void cih_oracle_read_linear(cih_oracle *c, float *out)
@@ -1691,6 +2108,28 @@ int cih_oracle_replay(cih_oracle *c, const void *stream, size_t bytes)
             break;
         }
         case CIH_OP_SET_FILL_RULE: c->evenodd = ia(a, 0) == 1; break;
+        case CIH_OP_SET_FONT: {
+            int n = ia(a, 0);
+            set_font(c, n ? (const unsigned char *)(a + 2) : NULL, n, fa(a, 1));
+            break;
+        }
+        case CIH_OP_FILL_TEXT:
+        case CIH_OP_STROKE_TEXT: {
+            size_t n = (size_t)ia(a, 0);
+            char *text = (char *)malloc(n + 1);
+            if (!text)
+                goto malformed;
+            memcpy(text, a + 4, n);
+            text[n] = 0;
+            if (r.op == CIH_OP_FILL_TEXT)
+                fill_text(c, text, fa(a, 1), fa(a, 2), fa(a, 3));
+            else
+                stroke_text(c, text, fa(a, 1), fa(a, 2), fa(a, 3));
+            free(text);
+            break;
+        }
+        case CIH_OP_SET_TEXT_ALIGN: c->text_align = ia(a, 0); break;
+        case CIH_OP_SET_TEXT_BASELINE: c->text_baseline = ia(a, 0); break;
         default: goto malformed;
         }
     }

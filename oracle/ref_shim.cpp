@@ -150,6 +150,17 @@ long cih_ref_path(void *handle, float *points, long capacity, int *subpath_info,
     return n;
 }
 
+// The reference's set_font (hpp:3205-3274) on a fresh canvas: whether it
+// accepts the font, and measure_text (hpp:3297-3311) after it.
+int cih_ref_set_font(const unsigned char *font, int bytes, float size, const char *text,
+                     float *width)
+{
+    canvas c(0, 0);
+    bool ok = c.set_font(font, bytes, size);
+    *width = c.measure_text(text);
+    return ok ? 1 : 0;
+}
+
 // The reference's is_point_in_path (hpp:3101-3132) on the current path.
 int cih_ref_is_point_in_path(void *handle, float x, float y)
 {

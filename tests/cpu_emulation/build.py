@@ -155,7 +155,7 @@ def emulated_runtime(src):
 
 def _sources():
     names = ["cih_kernels.hip", "cih_runtime.cpp", "cih_replay.cpp", "cih_scene.cpp",
-             "cih_device.hpp", "cih_geometry.hpp", "cih_blur.hpp"]
+             "cih_text.cpp", "cih_device.hpp", "cih_geometry.hpp", "cih_blur.hpp"]
     h = hashlib.sha256()
     for n in names:
         h.update(open(os.path.join(CSRC, n), "rb").read())
@@ -192,6 +192,7 @@ def build(force=False):
         (os.path.join(OUT, "emu_runtime.cpp"), ["-fpermissive", "-fvisibility=hidden", "-w"]),
         (os.path.join(CSRC, "cih_replay.cpp"), ["-fvisibility=hidden"]),
         (os.path.join(CSRC, "cih_scene.cpp"), ["-fvisibility=hidden"]),
+        (os.path.join(CSRC, "cih_text.cpp"), ["-fvisibility=hidden"]),
         (os.path.join(HERE, "emu_rt.cpp"), ["-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0"]),
     ]
     procs, objects = [], []

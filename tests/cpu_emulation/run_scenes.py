@@ -28,6 +28,10 @@ def streams():
     }
     out.update({"clip_" + k: v for k, v in scenes.clip_scenes().items()})
     out.update({"reference_" + k: v for k, v in scenes.reference_clip_tests(64).items()})
+    measure = lambda font, size, text: oracle.measure_text(font, size, text)
+    out.update({"text_" + k: v
+                for k, v in scenes.reference_text_tests(128, measure=measure).items()})
+    out.update({"text_scene_%d" % k: scenes.text_scene(k, 96) for k in range(4)})
     return out
 
 

@@ -8,7 +8,9 @@ Porter-Duff operator table of the W3C 2D canvas specification, the IEC
 61966-2-1 sRGB transfer, linear interpolation of gradient stops, mass
 conservation of a normalised blur, identity resampling, stroke areas.
 """
+import json
 import math
+import os
 
 import numpy as np
 
@@ -487,3 +489,277 @@ def reference_clip_tests(size=256):
     s.save(); s.save()
     out["save_restore"] = s.tobytes()
     return out
+
+
+# ---- the reference test suite's text tests and image hash -----------------
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def font(name):
+    """A TrueType fixture from the reference's tests (tests/golden/fonts)."""
+    with open(os.path.join(GOLDEN, "fonts", "font_%s.ttf" % name), "rb") as f:
+        return f.read()
+
+
+def reference_hashes():
+    """{test name: {"hash", "width", "height"}} from the reference's table."""
+    with open(os.path.join(GOLDEN, "reference_hashes.json")) as f:
+        return json.load(f)
+
+
+_HASH_ROLLS = {}
+
+
+def _hash_rolls(count):
+    """The rotation of each (pixel, channel) step of the hash's xorshift state
+    (test.cpp:2370, 2397-2402); it does not depend on the image."""
+    if count not in _HASH_ROLLS:
+        state, rolls = 0xFFFFFFFF, np.empty(count, dtype=np.uint32)
+        for k in range(count):
+            state ^= (state & 0x7FFFF) << 13
+            state ^= state >> 17
+            state ^= (state & 0x7FFFFFF) << 5
+            state &= 0xFFFFFFFF
+            rolls[k] = state >> 27
+        _HASH_ROLLS[count] = rolls
+    return _HASH_ROLLS[count]
+
+
+def image_hash(rgba8):
+    """The reference suite's 32-bit locality-sensitive image hash
+    (test.cpp:2365-2406): per channel, edge flags against the right and lower
+    neighbours (wrapping; colour channels weighted by alpha), each rotated by
+    a pseudorandom amount and XORed together."""
+    img = np.asarray(rgba8, dtype=np.int64)
+    h, w = img.shape[:2]
+    alpha = img[:, :, 3:4]
+    weighted = np.concatenate([img[:, :, :3] * alpha, img[:, :, 3:4]], axis=2)
+    threshold = np.array([8 * 255] * 3 + [8], dtype=np.int64)
+    down = np.roll(weighted, -1, axis=0)
+    right = np.roll(weighted, -1, axis=1)
+
+    def flags(d, hi, lo):
+        return (np.where(d > threshold * 16, hi, 0) | np.where(d > threshold, lo, 0))
+    edges = (flags(weighted - down, 128, 64) | flags(down - weighted, 32, 16) |
+             flags(weighted - right, 8, 4) | flags(right - weighted, 2, 1))
+    edges = edges.reshape(-1).astype(np.uint64)
+    rolls = _hash_rolls(edges.size).astype(np.uint64)
+    rotated = ((edges << rolls) | (edges >> ((32 - rolls) % 32) * (rolls != 0))) & 0xFFFFFFFF
+    return int(np.bitwise_xor.reduce(rotated))
+
+
+def hash_distance(a, b):
+    return bin((a ^ b) & 0xFFFFFFFF).count("1")
+
+
+def _f(v):
+    return np.float32(v)
+
+
+def reference_text_tests(size=256, measure=None):
+    """The call sequences of the reference's text tests (test.cpp:1604-1741
+    text_align, text_baseline, font, fill_text, stroke_text, measure_text and
+    1875-1907 example_button) as command streams, with float32 arithmetic
+    for every computed coordinate.  `measure(font, size, text)` supplies
+    measure_text's values, which the reference test adds up to place glyphs."""
+    w = h = _f(size)
+    fa, fb, fc = font("a"), font("b"), font("c")
+    out = {}
+    line = "CE\xc3\x8d\xf4\x8f\xbf\xbd\xf0I".encode("latin-1")
+
+    def guides(s):
+        s.set_color(1, 1.0, 0.0, 0.0, 0.5)
+        s.set_line_width(1.0)
+        s.move_to(0.0, _f(0.5) * h); s.line_to(w, _f(0.5) * h)
+        s.move_to(_f(0.5) * w, 0.0); s.line_to(_f(0.5) * w, h)
+        s.stroke()
+
+    s = Stream(size, size)
+    s.set_font(fa, _f(0.2) * h)
+    s.rotate(0.2)
+    s.set_color(0, 0.0, 0.0, 0.0, 1.0)
+    for index, align in enumerate((0, 2, 1, 0, 1)):  # leftward center rightward start ending
+        base = (_f(0.1) + _f(0.2) * _f(index)) * h
+        s.set_text_align(align)
+        s.fill_text("HIty", _f(0.5) * w, base)
+    guides(s)
+    out["text_align"] = s.tobytes()
+
+    s = Stream(size, size)
+    s.set_font(fa, _f(0.2) * h)
+    s.rotate(0.2)
+    s.set_color(0, 0.0, 0.0, 0.0, 1.0)
+    for index, baseline in enumerate((0, 1, 2, 3, 4, 3)):  # ... hanging ideographic
+        left = (_f(0.1) + _f(0.15) * _f(index)) * w
+        s.set_text_baseline(baseline)
+        s.fill_text("Iy", left, _f(0.5) * h)
+    guides(s)
+    out["text_baseline"] = s.tobytes()
+
+    s = Stream(size, size)
+    s.set_color(1, 1.0, 0.0, 0.0, 1.0)
+    s.stroke_text("D", _f(0.8) * w, _f(0.95) * h)
+    s.set_color(0, 1.0, 0.0, 0.0, 1.0)
+    s.fill_text("D", _f(0.9) * w, _f(0.95) * h)
+    s.set_font(None, _f(0.1) * h)
+    s.set_color(0, 0.0, 0.0, 0.0, 1.0)
+    s.set_font(fa, _f(0.2) * h)
+    s.fill_text(line, 0.0, _f(0.20) * h)
+    s.set_font(None, _f(0.1) * h)
+    s.fill_text(line, _f(0.65) * w, _f(0.20) * h)
+    s.set_font(fb, _f(0.2) * h)
+    s.fill_text(line, 0.0, _f(0.45) * h)
+    s.set_font(fc, _f(0.2) * h)
+    s.fill_text(line, 0.0, _f(0.70) * h)
+    s.set_color(0, 1.0, 0.0, 0.0, 1.0)
+    for name, x in (("d", 0.1), ("e", 0.2), ("f", 0.3), ("g", 0.4)):
+        s.set_font(font(name), _f(0.2) * h)
+        s.fill_text("D", _f(x) * w, _f(0.95) * h)
+    out["font"] = s.tobytes()
+
+    s = Stream(size, size)
+    s.set_linear_gradient(0, _f(0.4) * w, 0.0, _f(0.6) * w, 0.0)
+    for stop in ((0.00, 0.0, 0.00, 1.0, 1.0), (0.45, 0.0, 0.25, 0.5, 1.0),
+                 (0.50, 1.0, 0.00, 0.0, 1.0), (0.55, 0.0, 0.25, 0.5, 1.0),
+                 (1.00, 0.0, 0.50, 0.0, 1.0)):
+        s.add_color_stop(0, *stop)
+    s.set_font(fa, _f(0.3) * h)
+    s.rotate(0.2)
+    s.set_shadow_offset(2.0, 2.0)
+    s.set_shadow_blur(4.0)
+    s.set_shadow_color(0.0, 0.0, 0.0, 0.75)
+    s.move_to(0.0, _f(0.2) * h)
+    s.fill_text("Canvas", _f(0.1) * w, _f(0.2) * h)
+    s.line_to(w, _f(0.2) * h)
+    s.fill_text("Ity\n*", _f(0.2) * w, _f(0.5) * h, w)
+    s.move_to(0.0, _f(0.5) * h)
+    s.fill_text("*Canvas\fIty*", _f(0.2) * w, _f(0.8) * h, _f(0.7) * w)
+    s.set_color(0, 1.0, 0.0, 0.0, 1.0)
+    s.fill_text("****", _f(0.1) * w, _f(0.35) * h, 0.0)
+    s.line_to(w, _f(0.5) * h)
+    s.set_shadow_color(0.0, 0.0, 0.0, 0.0)
+    s.set_color(1, 1.0, 0.0, 0.0, 1.0)
+    s.set_line_width(2.0)
+    s.stroke()
+    out["fill_text"] = s.tobytes()
+
+    s = Stream(size, size)
+    s.set_color(1, 0.0, 0.0, 0.0, 1.0)
+    s.set_font(fa, _f(0.3) * h)
+    s.rotate(0.2)
+    s.set_line_width(2.0)
+    s.set_line_dash([8.0, 2.0])
+    s.move_to(0.0, _f(0.2) * h)
+    s.stroke_text("Canvas", _f(0.1) * w, _f(0.2) * h)
+    s.line_to(w, _f(0.2) * h)
+    s.stroke_text("Ity\n*", _f(0.2) * w, _f(0.5) * h, w)
+    s.move_to(0.0, _f(0.5) * h)
+    s.stroke_text("*Canvas\fIty*", _f(0.2) * w, _f(0.8) * h, _f(0.7) * w)
+    s.set_color(1, 1.0, 0.0, 0.0, 1.0)
+    s.stroke_text("****", _f(0.1) * w, _f(0.35) * h, 0.0)
+    s.line_to(w, _f(0.5) * h)
+    s.set_line_dash(None)
+    s.stroke()
+    out["stroke_text"] = s.tobytes()
+
+    if measure is not None:
+        s = Stream(size, size)
+        s.set_color(0, 0.0, 0.0, 0.0, 1.0)
+        place = _f(0.1) * w
+        place = _f(place + _f(0.0))  # measure_text("C") before any font: 0
+        size_a = _f(0.3) * h
+        s.set_font(fa, size_a)
+        s.rotate(0.5)
+        s.scale(1.15, 1.0)
+        for text, y in (("C", 0.2), ("a", 0.25), ("nv", 0.2), ("a", 0.15), ("s", 0.2)):
+            s.fill_text(text, place, _f(y) * h)
+            place = _f(place + _f(measure(fa, size_a, text)))
+        out["measure_text"] = s.tobytes()
+
+    s = Stream(size, size)
+    def roundf(v):  # halves away from zero; the values here are positive
+        return _f(math.floor(float(v) + 0.5))
+    left, right = roundf(_f(0.25) * w), roundf(_f(0.75) * w)
+    top, bottom = roundf(_f(0.375) * h), roundf(_f(0.625) * h)
+    mid_x, mid_y = (left + right) * _f(0.5), (top + bottom) * _f(0.5)
+    half = _f(0.5)
+    s.set_shadow_offset(3.0, 3.0)
+    s.set_shadow_blur(3.0)
+    s.set_shadow_color(0.0, 0.0, 0.0, 0.5)
+    s.set_linear_gradient(0, 0.0, top, 0.0, bottom)
+    s.add_color_stop(0, 0.0, 0.3, 0.3, 0.3, 1.0)
+    s.add_color_stop(0, 1.0, 0.2, 0.2, 0.2, 1.0)
+    s.move_to(left + half, mid_y)
+    s.arc_to(left + half, top + half, mid_x, top + half, 4.0)
+    s.arc_to(right - half, top + half, right - half, mid_y, 4.0)
+    s.arc_to(right - half, bottom - half, mid_x, bottom - half, 4.0)
+    s.arc_to(left + half, bottom - half, left + half, mid_y, 4.0)
+    s.close_path()
+    s.fill()
+    s.set_shadow_color(0.0, 0.0, 0.0, 0.0)
+    s.set_font(fa, _f(0.075) * h)
+    s.set_text_align(2)     # center
+    s.set_text_baseline(2)  # middle
+    s.set_color(0, 0.8, 0.8, 0.8, 1.0)
+    s.fill_text("* Cats", _f(0.5) * w, _f(0.5) * h)
+    s.set_color(0, 0.4, 0.4, 0.4, 1.0)
+    s.fill_rectangle(left + _f(4.0), top + _f(1.0), right - left - _f(8.0), _f(1.0))
+    s.set_color(1, 0.1, 0.1, 0.1, 1.0)
+    s.stroke()
+    out["example_button"] = s.tobytes()
+    return out
+
+
+TEXT_CHARS = ["C", "D", "E", "F", "G", "H", "I", "a", "n", "s", "t", "v", "y", "*", " ", "\n",
+              "\t", "Í", "\U0010fffd", "?", "Q"]
+
+
+def text_scene(seed, size=160):
+    """A seeded text scene: fill_text and stroke_text of random strings over
+    the reference's three valid test fonts, with random transforms, alignment,
+    baselines, maximum widths, dashes, shadows, gradients and composite
+    operations.  Characters the fonts lack draw glyph 0."""
+    rng = np.random.default_rng(seed)
+    u = lambda lo, hi: float(np.float32(rng.uniform(lo, hi)))
+    s = Stream(size, size)
+    fonts = [font("a"), font("b"), font("c")]
+    ops = sorted(COMPOSITE.values())
+    for k in range(int(rng.integers(3, 7))):
+        s.save()
+        s.translate(u(0, size * 0.5), u(size * 0.2, size * 0.8))
+        s.rotate(u(-0.8, 0.8))
+        if rng.integers(0, 3) == 0:
+            s.scale(u(0.6, 1.5), u(0.6, 1.5))
+        s.set_font(fonts[int(rng.integers(0, 3))], u(8, size * 0.3))
+        s.set_text_align(int(rng.integers(0, 3)))
+        s.set_text_baseline(int(rng.integers(0, 5)))
+        if rng.integers(0, 4) == 0:
+            s.set_linear_gradient(int(rng.integers(0, 2)), 0.0, 0.0, u(20, size), u(0, 40))
+            for t in sorted(rng.uniform(0, 1, size=int(rng.integers(2, 5)))):
+                s.add_color_stop(0, float(t), u(0, 1), u(0, 1), u(0, 1), u(0.5, 1))
+                s.add_color_stop(1, float(t), u(0, 1), u(0, 1), u(0, 1), u(0.5, 1))
+        else:
+            s.set_color(0, u(0, 1), u(0, 1), u(0, 1), u(0.4, 1))
+            s.set_color(1, u(0, 1), u(0, 1), u(0, 1), u(0.4, 1))
+        if rng.integers(0, 4) == 0:
+            s.set_shadow_color(u(0, 1), u(0, 1), u(0, 1), u(0.3, 0.9))
+            s.set_shadow_offset(u(-6, 6), u(-6, 6))
+            s.set_shadow_blur(u(0, 6))
+        if rng.integers(0, 5) == 0:
+            s.set_composite_op(ops[int(rng.integers(0, len(ops)))])
+        text = "".join(TEXT_CHARS[int(i)] for i in rng.integers(0, len(TEXT_CHARS),
+                                                                  size=int(rng.integers(1, 9))))
+        maximum = [1.0e30, u(5, size), 0.0][int(rng.choice(3, p=[0.6, 0.3, 0.1]))]
+        if rng.integers(0, 2):
+            s.fill_text(text, 0.0, 0.0, maximum)
+        else:
+            s.set_line_width(u(0.5, 5))
+            s.set_line_join(int(rng.integers(0, 3)))
+            s.set_line_cap(int(rng.integers(0, 3)))
+            if rng.integers(0, 3) == 0:
+                s.set_line_dash([u(1, 12) for _ in range(int(rng.integers(1, 4)))])
+                s.set_line_dash_offset(u(0, 10))
+            s.stroke_text(text, 0.0, 0.0, maximum)
+        s.restore()
+    return s.tobytes()

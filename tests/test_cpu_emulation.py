@@ -54,3 +54,7 @@ def test_configurations_memory_safe_and_in_parity(library):
 def test_clip_masks_memory_safe_and_in_parity(library):
     run(library, ["clip_nested", "clip_op_destination_in", "clip_shadow_image",
                   "clip_star_empty", "reference_global_composite_operation"])
+
+
+def test_text_memory_safe_and_in_parity(library):
+    run(library, ["text_font", "text_fill_text", "text_stroke_text", "text_scene_1"])

@@ -164,6 +164,56 @@ def test_clip_in_batches_and_server(device):
         assert np.array_equal(alone[i][0], got[rid]), i
 
 
+# ---- text (hpp:1526-1846, 3205-3311) -------------------------------------------
+
+TEXT_TESTS = ["text_align", "text_baseline", "font", "fill_text", "stroke_text",
+              "measure_text", "example_button"]
+
+
+def reference_text_tests():
+    return scenes.reference_text_tests(256, measure=oracle.measure_text)
+
+
+@pytest.mark.parametrize("name", TEXT_TESTS)
+def test_reference_text_test(device, name):
+    # the reference's text tests (test.cpp) against the oracle, and against the
+    # reference suite's own pass criterion: image hash within Hamming
+    # distance 5 of its golden table (test.cpp:2614-2618)
+    (r8, rl), (g8, gl) = both(reference_text_tests()[name])
+    assert_parity(r8, rl, g8, gl, name)
+    golden = scenes.reference_hashes()[name]["hash"]
+    assert scenes.hash_distance(scenes.image_hash(g8), golden) <= 5
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_text_scene(device, seed):
+    check(scenes.text_scene(seed), "text scene %d" % seed)
+
+
+def test_text_in_batches_and_server(device):
+    # text canvases in one batch and through the server: the same pixels as
+    # each rendered alone
+    tests = reference_text_tests()
+    streams = [tests[n] for n in TEXT_TESTS] + [scenes.text_scene(s) for s in range(4)]
+    alone = [cia.render(s, device) for s in streams]
+    b = cia.Batch(streams, device)
+    b.execute(to_host=True)
+    for i in range(len(streams)):
+        assert np.array_equal(alone[i][1].view(np.uint32), b.linear(i).view(np.uint32)), i
+        assert np.array_equal(alone[i][0], b.rgba8(i)), i
+    b.close()
+    server = cia.Server(hbm_budget=1 << 30, max_batch=4, device=device)
+    ids = [server.submit(s) for s in streams]
+    got = {}
+    while len(got) < len(ids):
+        rid = server.wait(timeout_ms=60000)
+        assert rid is not None, "server stalled"
+        got[rid] = server.result(rid)[0]
+    server.stop()
+    for i, rid in enumerate(ids):
+        assert np.array_equal(alone[i][0], got[rid]), i
+
+
 def test_plain_512_full(device):
     m = check(cia.scene(1, 0xC1), "plain-512")
     print("plain-512", m)
