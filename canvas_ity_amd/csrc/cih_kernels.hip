@@ -2176,7 +2176,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
             // row); operators without bit 8 repaint every pixel (hpp:2574).
             need_main = !(op & 8) || has_steps ||
                         coverage(before, (flags & CIH_DRAW_FLAG_EVENODD) != 0) >= kThreshold;
-            if (D.kind == CIH_DRAW_CLIP) // makes a mask, paints nothing
+            if (D.kind == CIH_DRAW_CLIP || D.kind == CIH_DRAW_PUT_IMAGE) // k_clip / k_put
                 need_main = need_shadow = false;
             if (D.brush == CIH_BRUSH_COLOR && D.ncolors)
                 color = load4((const float *)(F.payload + D.colors_at));
@@ -2428,6 +2428,32 @@ namespace cih {
 
 int fine_segment() { return FINE_SEGMENT; }
 
+static unsigned grid_for(size_t n, unsigned block) {
+    size_t g = (n + block - 1) / block;
+    return (unsigned)(g < 65536 ? (g ? g : 1) : 65536);
+}
+
+// put_image_data of draw d (hpp:3383-3408): the image's texels (already
+// premultiplied linear, as put_image_data converts them) replace the canvas
+// pixels they land on.  A grid-stride loop over the image.
+__global__ void k_put(frame F, int d) {
+    const draw_desc &D = F.draws[d];
+    const canvas_desc &C = F.canvases[D.canvas];
+    const image_desc &I = F.images[D.image];
+    const float4 *texels = (const float4 *)(uintptr_t)I.texels;
+    float4 *bitmap = (float4 *)(uintptr_t)C.bitmap;
+    const int w = D.image_width, h = D.image_height;
+    const long x0 = (long)D.start_x, y0 = (long)D.start_y;
+    const size_t n = (size_t)w * (size_t)h;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        long cx = x0 + (long)(i % (size_t)w), cy = y0 + (long)(i / (size_t)w);
+        if (cx < 0 || cx >= C.width || cy < 0 || cy >= C.height)
+            continue;
+        bitmap[(size_t)cy * (size_t)C.width + (size_t)cx] = texels[i];
+    }
+}
+
 const int kFrameKernels = 8;
 extern const char *const kFrameKernelNames[kFrameKernels] = {
     "k_reset", "k_geometry", "k_runs", "k_coalesce", "k_shadow_raster", "k_blur_rows",
@@ -2437,8 +2463,8 @@ extern const char *const kFrameKernelNames[kFrameKernels] = {
 // of them), event k is recorded before kernel k and the last after the last
 // kernel, so the time of kernel k is elapsed(events[k], events[k + 1]).
 hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool any_shadow,
-                        const int32_t *clips, int nclips, bool masked, hipEvent_t *events,
-                        hipStream_t stream) {
+                        const int32_t *clips, int nclips, const int32_t *puts, int nputs,
+                        bool masked, hipEvent_t *events, hipStream_t stream) {
 #define MARK(k)                                                                                \
     do {                                                                                       \
         if (events)                                                                            \
@@ -2446,6 +2472,10 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
     } while (0)
     MARK(0);
     k_reset<<<1, 64, 0, stream>>>(F.counters);
+    // puts in draw order (puts[2k] = draw, puts[2k + 1] = texels): a frame
+    // holds no painting draw of a canvas before its puts
+    for (int k = 0; k < nputs; ++k)
+        k_put<<<grid_for((size_t)puts[2 * k + 1], 256), 256, 0, stream>>>(F, puts[2 * k]);
     MARK(1);
     if (F.ndraws > 0)
         k_geometry<<<F.ndraws, GEO_THREADS, 0, stream>>>(F);
@@ -2486,11 +2516,6 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
     MARK(8);
 #undef MARK
     return hipGetLastError();
-}
-
-static unsigned grid_for(size_t n, unsigned block) {
-    size_t g = (n + block - 1) / block;
-    return (unsigned)(g < 65536 ? (g ? g : 1) : 65536);
 }
 
 hipError_t launch_clear(void *pixels, size_t n, hipStream_t stream) {

@@ -56,8 +56,8 @@ const uint32_t kMaxMasks = 256;
 const uint64_t kMaskBytesLimit = (uint64_t)8 << 30;
 extern const char *const kFrameKernelNames[kFrameKernels];
 hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool any_shadow,
-                        const int32_t *clips, int nclips, bool masked,
-                        hipEvent_t *events, hipStream_t stream);
+                        const int32_t *clips, int nclips, const int32_t *puts, int nputs,
+                        bool masked, hipEvent_t *events, hipStream_t stream);
 hipError_t launch_clear(void *pixels, size_t n, hipStream_t stream);
 hipError_t launch_prepare_image(const void *rgba8, void *texels, size_t n, const float *lut,
                                 hipStream_t stream);
@@ -228,6 +228,10 @@ struct cih_canvas {
     int width, height;
     void *pixels = nullptr; // float4 bitmap in HBM (owned by `bitmap` or by a batch)
     dbuf bitmap;
+    // Batch and serving canvases record put_image_data as a packet in draw
+    // order (their pixels are drawn later, from cleared bitmaps); other
+    // canvases apply it at once.
+    bool deferred = false;
     std::vector<unsigned char> pending;
     program program;
     // clip masks: a table of kMaxMasks mask_desc in HBM (entry 0 unused: the
@@ -256,7 +260,14 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
                   4 * (size_t)p.dashes + 16 * (size_t)p.colors + 4 * (size_t)p.stops;
     if (p.bytes < body || p.bytes > end - at || (p.bytes & 15))
         return 0;
-    if (p.kind < CIH_DRAW_FILL_PATH || p.kind > CIH_DRAW_CLIP)
+    if (p.kind < CIH_DRAW_FILL_PATH || p.kind > CIH_DRAW_PUT_IMAGE)
+        return 0;
+    // a put: an image and an integer position, nothing else
+    if (p.kind == CIH_DRAW_PUT_IMAGE &&
+        (p.points || p.subpaths || p.dashes || p.colors || p.stops || p.flags ||
+         p.brush != CIH_BRUSH_IMAGE || p.clip_mask || !(fabsf(p.start_x) <= 16777216.0f) ||
+         !(fabsf(p.start_y) <= 16777216.0f) || p.start_x != floorf(p.start_x) ||
+         p.start_y != floorf(p.start_y)))
         return 0;
     if (p.clip_mask >= kMaxMasks ||
         (p.kind == CIH_DRAW_CLIP && (p.image < 1 || p.image >= (int32_t)kMaxMasks)))
@@ -308,7 +319,10 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
     std::memcpy(out.shadow_color, p.shadow_color, sizeof(out.shadow_color));
     out.shadow_offset_x = p.shadow_offset_x;
     out.shadow_offset_y = p.shadow_offset_y;
-    out.shadow = p.shadow_color[3] != 0.0f && p.kind != CIH_DRAW_CLIP ? 1 : 0;
+    out.shadow = p.shadow_color[3] != 0.0f && p.kind != CIH_DRAW_CLIP &&
+                         p.kind != CIH_DRAW_PUT_IMAGE
+                     ? 1
+                     : 0;
     out.clip_mask = (int32_t)p.clip_mask;
     // Extended-box blur constants (hpp:2402-2405, 2453-2459), in float as
     // the reference computes them.
@@ -394,6 +408,7 @@ struct frame_shape {
     size_t ndraws = 0;
     std::vector<uint32_t> items; // per-canvas prefix of (row, segment) items
     std::vector<int32_t> clips;  // (draw, canvas height) of each clip draw, in order
+    std::vector<int32_t> puts;   // (draw, texel count) of each put, in order
     bool masked = false;         // some draw is composited under a clip mask
 };
 
@@ -412,6 +427,10 @@ frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t 
             if (dd.kind == CIH_DRAW_CLIP) {
                 f.clips.push_back(k);
                 f.clips.push_back(cd.height);
+            }
+            if (dd.kind == CIH_DRAW_PUT_IMAGE) {
+                f.puts.push_back(k);
+                f.puts.push_back(dd.image_width * dd.image_height);
             }
             f.masked = f.masked || (dd.clip_mask != 0 && dd.kind != CIH_DRAW_CLIP);
             if (dd.shadow) {
@@ -481,9 +500,9 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
                 return s;
         }
         HIP_TRY(launch_frame(F, shape.max_rows, shape.max_cols, shape.any_shadow,
-                             shape.clips.data(), (int)(shape.clips.size() / 2), shape.masked,
-                             events,
-                             L.stream));
+                             shape.clips.data(), (int)(shape.clips.size() / 2),
+                             shape.puts.data(), (int)(shape.puts.size() / 2), shape.masked,
+                             events, L.stream));
         d->frames++;
         if (!check)
             return CIH_OK;
@@ -585,17 +604,24 @@ int note_masks(cih_canvas *cv, const draw_desc &desc) {
 // Whether a frame must end before draw `dd` of canvas `canvas`: a clip that
 // redefines a mask id the frame already uses on that canvas (a frame builds
 // all its masks before compositing, so each id must name one mask within
-// it).  `used` and `current` track the frame; they reset at a new canvas.
-bool mask_cut(const draw_desc &dd, int canvas, std::vector<unsigned char> &used, int &current) {
+// it), or a put on a canvas that already has painting draws in the frame (a
+// frame applies its puts before compositing).  `used` (kCutSlots entries)
+// and `current` track the frame; they reset at a new canvas.
+const size_t kCutSlots = kMaxMasks + 1; // mask ids, then "the canvas has painted"
+bool frame_cut(const draw_desc &dd, int canvas, std::vector<unsigned char> &used, int &current) {
     if (canvas != current) {
         std::fill(used.begin(), used.end(), (unsigned char)0);
         current = canvas;
     }
+    if (dd.kind == CIH_DRAW_PUT_IMAGE)
+        return used[kMaxMasks] != 0;
     if (dd.kind == CIH_DRAW_CLIP && used[(uint32_t)dd.image])
         return true;
     used[(uint32_t)dd.clip_mask] = 1;
     if (dd.kind == CIH_DRAW_CLIP)
         used[(uint32_t)dd.image] = 1;
+    else
+        used[kMaxMasks] = 1;
     return false;
 }
 
@@ -634,18 +660,17 @@ int prepare_program(cih_canvas *cv) {
     HIP_TRY(hipMemcpyAsync(P.draws.ptr, stage + bytes.size(), sizeof(draw_desc) * P.descs.size(),
                            hipMemcpyHostToDevice, d->stream));
     // Frames of bounded size: shadow images are the largest intermediates.
-    // A frame also ends before a clip that redefines a mask id the frame
-    // already uses: a frame builds all its masks before compositing, so each
-    // id must name one mask within it.
+    // A frame also ends where frame_cut says (mask id reuse, puts after
+    // painting draws).
     const uint64_t shadow_budget = (uint64_t)1 << 30; // floats
-    std::vector<unsigned char> used(kMaxMasks);
+    std::vector<unsigned char> used(kCutSlots);
     size_t first = 0;
     while (first < P.descs.size()) {
         size_t last = first;
         uint64_t shadow = 0;
         int current = -1;
         while (last < P.descs.size() && last - first < 8192) {
-            if (mask_cut(P.descs[last], 0, used, current) && last > first)
+            if (frame_cut(P.descs[last], 0, used, current) && last > first)
                 break;
             uint64_t need = 0;
             if (P.descs[last].shadow) {
@@ -984,6 +1009,34 @@ int cih_canvas_put_image_data(cih_canvas *c, const unsigned char *image, int wid
     cih_device *d = c->device;
     std::lock_guard<std::recursive_mutex> lock(d->mutex);
     int s;
+    if (c->deferred) {
+        // the texels become a resident device image; the packet puts them
+        // where painter's order reaches it
+        int32_t id = -1;
+        if ((s = cih_image_create(d, image, width, height, stride, &id)))
+            return s;
+        cih_draw_packet p;
+        std::memset(&p, 0, sizeof(p));
+        p.bytes = (uint32_t)sizeof(p);
+        p.kind = CIH_DRAW_PUT_IMAGE;
+        p.op = 14; // source_over: no whole-canvas effect
+        p.global_alpha = 1.0f;
+        p.brush = CIH_BRUSH_IMAGE;
+        p.image = id;
+        p.image_width = width;
+        p.image_height = height;
+        // positions past 2^20 put nothing on a canvas of at most 32768^2
+        const int far = 1 << 20;
+        p.start_x = (float)std::min(std::max(x, -far), far);
+        p.start_y = (float)std::min(std::max(y, -far), far);
+        try {
+            const unsigned char *b = (const unsigned char *)&p;
+            c->pending.insert(c->pending.end(), b, b + sizeof(p));
+        } catch (const std::bad_alloc &) {
+            return fail(CIH_ERROR_OUT_OF_MEMORY, "out of host memory");
+        }
+        return CIH_OK;
+    }
     if ((s = cih_canvas_flush(c)))
         return s;
     size_t n = (size_t)width * (size_t)height;
@@ -1364,6 +1417,7 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
             }
             cih_canvas *v = new cih_canvas;
             v->device = d;
+            v->deferred = true;
             v->width = r.width;
             v->height = r.height;
             b->views.push_back(v);
@@ -1453,7 +1507,7 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
         const uint64_t shadow_budget = (uint64_t)1 << 30;
         std::vector<canvas_desc> frame_canvases;
         std::vector<uint32_t> frame_items;
-        std::vector<unsigned char> used(kMaxMasks);
+        std::vector<unsigned char> used(kCutSlots);
         size_t draw = 0;
         while (draw < b->descs.size()) {
             size_t last = draw;
@@ -1461,7 +1515,7 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
             int current = -1;
             while (last < b->descs.size() && last - draw < max_draws) {
                 const draw_desc &dd = b->descs[last];
-                if (mask_cut(dd, dd.canvas, used, current) && last > draw)
+                if (frame_cut(dd, dd.canvas, used, current) && last > draw)
                     break;
                 uint64_t need = 0;
                 if (dd.shadow) {
@@ -1786,6 +1840,7 @@ void server_worker(cih_server *sv) {
         int st = view ? CIH_OK : CIH_ERROR_OUT_OF_MEMORY;
         if (view) {
             view->device = d;
+            view->deferred = true;
             view->width = r->width;
             view->height = r->height;
             st = cih_canvas_replay(view, r->stream.data(), r->stream.size());
@@ -1861,14 +1916,14 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
     size_t draw = 0;
     const size_t max_draws = 32768;
     const uint64_t shadow_budget = (uint64_t)1 << 30; // shadow floats per frame
-    std::vector<unsigned char> used(kMaxMasks);
+    std::vector<unsigned char> used(kCutSlots);
     while (draw < b->descs.size()) {
         size_t last = draw;
         uint64_t shadow = 0;
         int current = -1;
         while (last < b->descs.size() && last - draw < max_draws) {
             const draw_desc &dd = b->descs[last];
-            if (mask_cut(dd, dd.canvas, used, current) && last > draw)
+            if (frame_cut(dd, dd.canvas, used, current) && last > draw)
                 break;
             uint64_t need = 0;
             if (dd.shadow) {

@@ -63,9 +63,16 @@ enum cih_draw_kind {
     CIH_DRAW_STROKE_PATH = 2, /* stroke(): path_to_lines(true)+stroke_lines  */
     CIH_DRAW_FILL_LINES = 3,  /* fill_rectangle / draw_image polylines       */
     CIH_DRAW_STROKE_LINES = 4, /* stroke_rectangle polylines + stroke_lines  */
-    CIH_DRAW_CLIP = 5          /* clip(): the path's fill coverage times the
+    CIH_DRAW_CLIP = 5,         /* clip(): the path's fill coverage times the
                                   mask `clip_mask` becomes mask `image`
                                   (hpp:3057-3099); paints nothing             */
+    CIH_DRAW_PUT_IMAGE = 6     /* put_image_data recorded in draw order (batch
+                                  and serving canvases): the texels of device
+                                  image `image` (image_width x image_height,
+                                  brush CIH_BRUSH_IMAGE) replace the pixels at
+                                  the integer position (start_x, start_y),
+                                  whatever the clip, alpha or operator
+                                  (hpp:3383-3408); no geometry, no shadow     */
 };
 
 enum cih_brush_kind {

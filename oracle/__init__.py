@@ -72,6 +72,9 @@ def _load(kind):
         fn.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_float, ctypes.c_char_p,
                        ctypes.POINTER(ctypes.c_float)]
     else:
+        fn = lib.cih_oracle_is_point_in_path
+        fn.restype = ctypes.c_int
+        fn.argtypes = [_P, ctypes.c_float, ctypes.c_float]
         fn = lib.cih_oracle_measure_text
         fn.restype = ctypes.c_float
         fn.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_char_p]

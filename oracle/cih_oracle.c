@@ -20,8 +20,8 @@
  * bounds-checked (out-of-range bytes read as 0), which only matters for
  * malformed fonts, where the reference's reads are undefined.
  *
- * Out of scope: is_point_in_path (answered by the library on the host and
- * checked against the reference directly).
+ * is_point_in_path (hpp:3101-3132) is restated for the test suite's own
+ * query-dependent tests.
  */
 #include "cih_oracle.h"
 #include "cih_stream.h"
@@ -1899,6 +1899,31 @@ void cih_oracle_put_image_data(cih_oracle *c, const unsigned char *image, int wi
                            image[index + 2] / 255.0f, image[index + 3] / 255.0f);
             c->bitmap[cy * c->size_x + cx] = premultiplied(linearized(color));
         }
+}
+
+int cih_oracle_is_point_in_path(cih_oracle *c, float x, float y) /* hpp:3101-3132 */
+{
+    size_t index, subpath = 0, beginning = 0, ending = 0;
+    int winding = 0;
+    path_to_lines(c, 0);
+    for (index = 0; index < c->lines.points.n; ++index) {
+        xy from, to;
+        while (index >= ending) {
+            beginning = ending;
+            ending += c->lines.subpaths.v[subpath++].count;
+        }
+        from = c->lines.points.v[index];
+        to = c->lines.points.v[index + 1 < ending ? index + 1 : beginning];
+        if ((from.y < y && y <= to.y) || (to.y < y && y <= from.y)) {
+            float side = dot(perpendicular(sub(to, from)), sub(mkxy(x, y), from));
+            if (side == 0.0f)
+                return 1;
+            winding += side > 0.0f ? 1 : -1;
+        } else if (from.y == y && y == to.y &&
+                   ((from.x <= x && x <= to.x) || (to.x <= x && x <= from.x)))
+            return 1;
+    }
+    return winding != 0;
 }
 
 float cih_oracle_measure_text(const unsigned char *font, size_t bytes, float size,

@@ -27,6 +27,8 @@ void cih_oracle_get_image_data(cih_oracle *canvas, unsigned char *image, int wid
                                int height, int stride, int x, int y);
 void cih_oracle_put_image_data(cih_oracle *canvas, const unsigned char *image, int width,
                                int height, int stride, int x, int y);
+/* is_point_in_path(x, y) on the canvas' current path: 1 inside or on it. */
+int cih_oracle_is_point_in_path(cih_oracle *canvas, float x, float y);
 /* measure_text of `text` after set_font(font, bytes, size) on a fresh canvas. */
 float cih_oracle_measure_text(const unsigned char *font, size_t bytes, float size,
                               const char *text);

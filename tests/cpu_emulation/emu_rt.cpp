@@ -14,9 +14,14 @@
 
 namespace emu {
 
+// A lane is a fiber that lives for the whole process: started once (the only
+// ucontext switch, which costs signal-mask system calls), it runs the kernel
+// body for a workgroup, parks at `idle`, and is restarted there for the next
+// workgroup with a plain _longjmp.
 struct fiber {
     ucontext_t uctx;
-    jmp_buf jb;
+    jmp_buf jb, idle;
+    bool alive = false;   // started once; parked at `idle` between workgroups
     bool started = false, done = false;
     idx3 tid;
     int wave = 0, lane = 0;
@@ -33,7 +38,7 @@ struct rendezvous {
 };
 
 static const size_t kStack = 512 * 1024;
-static std::vector<fiber> fibers;
+static fiber fibers[1024];
 static fiber *cur = nullptr;
 static jmp_buf sched_jb;
 static void *sched_fake = nullptr;
@@ -121,27 +126,30 @@ void block_barrier(int site) {
 
 static void fiber_entry() {
     __sanitizer_finish_switch_fiber(nullptr, &sched_bottom, &sched_size);
-    fiber *f = cur;
-    g_body(g_ctx);
-    f->done = true;
-    progress++;
-    rendezvous &w = waves[f->wave];
-    w.live--;
-    w.exited[f->lane] = true;
-    if (w.arrived && w.arrived == w.live)
-        complete(w);
-    block.live--;
-    if (block.arrived && block.arrived == block.live)
-        complete(block);
-    __sanitizer_start_switch_fiber(nullptr, sched_bottom, sched_size);
-    _longjmp(sched_jb, 1);
+    for (;;) {
+        fiber *f = cur;
+        g_body(g_ctx);
+        f->done = true;
+        progress++;
+        rendezvous &w = waves[f->wave];
+        w.live--;
+        w.exited[f->lane] = true;
+        if (w.arrived && w.arrived == w.live)
+            complete(w);
+        block.live--;
+        if (block.arrived && block.arrived == block.live)
+            complete(block);
+        if (!_setjmp(f->idle)) { // park until the next workgroup
+            __sanitizer_start_switch_fiber(&f->fake, sched_bottom, sched_size);
+            _longjmp(sched_jb, 1);
+        }
+        __sanitizer_finish_switch_fiber(cur->fake, &sched_bottom, &sched_size);
+    }
 }
 
 static void run_block(unsigned bx, unsigned by, unsigned bz) {
     g_block = {bx, by, bz};
     unsigned n = g_bdim.x * g_bdim.y * g_bdim.z;
-    if (fibers.size() < n)
-        fibers.resize(n);
     int nwaves = (int)((n + 63) / 64);
     for (int k = 0; k < nwaves; ++k) {
         waves[k] = rendezvous();
@@ -173,8 +181,13 @@ static void run_block(unsigned bx, unsigned by, unsigned bz) {
                 continue;
             cur = &f;
             if (!_setjmp(sched_jb)) {
-                if (!f.started) {
+                if (!f.started && f.alive) { // restart a parked lane
                     f.started = true;
+                    __sanitizer_start_switch_fiber(&sched_fake, f.stack, kStack);
+                    _longjmp(f.idle, 1);
+                } else if (!f.started) {
+                    f.started = true;
+                    f.alive = true;
                     getcontext(&f.uctx);
                     f.uctx.uc_stack.ss_sp = f.stack;
                     f.uctx.uc_stack.ss_size = kStack;

@@ -32,7 +32,30 @@ def streams():
     out.update({"text_" + k: v
                 for k, v in scenes.reference_text_tests(128, measure=measure).items()})
     out.update({"text_scene_%d" % k: scenes.text_scene(k, 96) for k in range(4)})
+    out.update({"put_scene_%d" % k: scenes.put_scene(k) for k in range(2)})
     return out
+
+
+def batch_and_server(device):
+    """put_image_data recorded in a batch and in server requests: each canvas
+    as rendered alone, where it is applied at once."""
+    streams = [scenes.put_scene(1, 32), scenes.put_scene(2, 40)]
+    alone = [cia.render(s, device) for s in streams]
+    b = cia.Batch(streams, device)
+    for _ in range(2):  # the checked run, then the captured graph
+        b.execute(to_host=True)
+        for i in range(len(streams)):
+            assert np.array_equal(alone[i][1].view(np.uint32), b.linear(i).view(np.uint32)), i
+    b.close()
+    server = cia.Server(hbm_budget=1 << 28, max_batch=2, device=device)
+    ids = [server.submit(s) for s in streams]
+    got = {}
+    while len(got) < len(ids):
+        rid = server.wait(timeout_ms=600000)
+        got[rid] = server.result(rid)[0]
+    server.stop()
+    for i, rid in enumerate(ids):
+        assert np.array_equal(alone[i][0], got[rid]), i
 
 
 def main():
@@ -40,6 +63,10 @@ def main():
     all_streams = streams()
     device = cia.default_device()
     failed = False
+    if sys.argv[2:] == ["batch_and_server"]:
+        batch_and_server(device)
+        print("batch_and_server ok", flush=True)
+        return 0
     for name in sys.argv[2:]:
         s = all_streams[name]
         g8, gl = cia.render(s, device)

@@ -763,3 +763,29 @@ def text_scene(seed, size=160):
             s.stroke_text(text, 0.0, 0.0, maximum)
         s.restore()
     return s.tobytes()
+
+
+def put_scene(seed, size=48):
+    """put_image_data mixed with draws: at the start, between draws,
+    overlapping puts, under a clip (puts ignore it), off the canvas, and
+    before a whole-canvas operator -- hpp:3383-3408 replaces pixels in
+    painter's order whatever the clip, alpha or operator."""
+    rng = np.random.default_rng(seed)
+    s = Stream(size, size)
+    s.define_image(0, rng.integers(0, 256, size=(10, 12, 4), dtype=np.uint8))
+    s.define_image(1, rng.integers(0, 256, size=(7, 5, 4), dtype=np.uint8))
+    s.put_image_data(0, -3, 4)
+    s.set_color(0, 1.0, 0.5, 0.0, 0.7)
+    s.fill_rectangle(5.0, 5.0, 20.0, 30.0)
+    s.put_image_data(1, 10, 10)
+    s.put_image_data(0, 12, 8)
+    s.begin_path()
+    s.rectangle(0.0, 0.0, 30.0, 30.0)
+    s.clip()
+    s.set_color(0, 0.0, 0.2, 1.0, 0.5)
+    s.fill_rectangle(8.0, 2.0, 30.0, 20.0)
+    s.put_image_data(1, size - 3, size - 2)
+    s.put_image_data(1, 1 << 22, 0)
+    s.set_composite_op(COMPOSITE["source_copy"])
+    s.fill_rectangle(20.0, 20.0, 10.0, 10.0)
+    return s.tobytes()

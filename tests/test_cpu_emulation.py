@@ -58,3 +58,10 @@ def test_clip_masks_memory_safe_and_in_parity(library):
 
 def test_text_memory_safe_and_in_parity(library):
     run(library, ["text_font", "text_fill_text", "text_stroke_text", "text_scene_1"])
+
+
+def test_put_image_data_in_batches_and_server(library):
+    # puts recorded as packets in batch and server canvases, applied in
+    # painter's order: the same pixels as the plain canvas, which applies them
+    # at once
+    run(library, ["batch_and_server"])

@@ -214,6 +214,38 @@ def test_text_in_batches_and_server(device):
         assert np.array_equal(alone[i][0], got[rid]), i
 
 
+# ---- put_image_data in draw order (hpp:3383-3408) ------------------------------
+
+@pytest.mark.parametrize("seed", range(3))
+def test_put_scene(device, seed):
+    check(scenes.put_scene(seed), "put scene %d" % seed)
+
+
+def test_put_image_data_in_batches_and_server(device):
+    # batch and server canvases record put_image_data as packets applied in
+    # painter's order; a plain canvas applies it at once: the same pixels
+    import reference_suite
+    streams = [scenes.put_scene(s) for s in range(3)] + [
+        reference_suite.stream("put_image_data"), reference_suite.stream("draw_image")]
+    alone = [cia.render(s, device) for s in streams]
+    b = cia.Batch(streams, device)
+    for _ in range(2):  # checked run, then the captured graph
+        b.execute(to_host=True)
+        for i in range(len(streams)):
+            assert np.array_equal(alone[i][1].view(np.uint32), b.linear(i).view(np.uint32)), i
+    b.close()
+    server = cia.Server(hbm_budget=1 << 30, max_batch=2, device=device)
+    ids = [server.submit(s) for s in streams]
+    got = {}
+    while len(got) < len(ids):
+        rid = server.wait(timeout_ms=60000)
+        assert rid is not None, "server stalled"
+        got[rid] = server.result(rid)[0]
+    server.stop()
+    for i, rid in enumerate(ids):
+        assert np.array_equal(alone[i][0], got[rid]), i
+
+
 def test_plain_512_full(device):
     m = check(cia.scene(1, 0xC1), "plain-512")
     print("plain-512", m)
