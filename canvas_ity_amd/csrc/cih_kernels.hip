@@ -2468,7 +2468,7 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
 #define MARK(k)                                                                                \
     do {                                                                                       \
         if (events)                                                                            \
-            hipEventRecord(events[k], stream);                                                 \
+            (void)hipEventRecord(events[k], stream);                                           \
     } while (0)
     MARK(0);
     k_reset<<<1, 64, 0, stream>>>(F.counters);
