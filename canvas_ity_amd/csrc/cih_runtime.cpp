@@ -369,6 +369,12 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
 static_assert(sizeof(draw_state) % 8 == 0, "the work list after the records is 8-byte aligned");
 size_t state_bytes(size_t n) { return (sizeof(draw_state) + 16) * std::max<size_t>(n, 1); }
 
+// Runs and steps share indices: a fallback allocation of the exact size can
+// leave one smaller than the other, so the smaller bounds both.
+uint64_t run_capacity(const exec_lane &L) {
+    return std::min(L.runs.bytes / sizeof(run), L.steps.bytes / sizeof(step));
+}
+
 int ensure_arenas(exec_lane &L, uint64_t pts, uint64_t words, uint64_t runs, uint64_t shadow) {
     int s;
     if ((s = L.points.reserve(8 * pts)) || (s = L.words.reserve(4 * words)) ||
@@ -465,7 +471,7 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
         uint64_t want_runs = std::max<uint64_t>(1 << 22, 16 * shape.in_points);
         if ((s = ensure_arenas(L, std::max<uint64_t>(want_points, L.points.bytes / 8),
                                std::max<uint64_t>(want_words, L.words.bytes / 4),
-                               std::max<uint64_t>(want_runs, L.runs.bytes / sizeof(run)),
+                               std::max<uint64_t>(want_runs, run_capacity(L)),
                                std::max<uint64_t>(shape.shadow_bound, L.shadow.bytes / 4))))
             return s;
     }
@@ -487,7 +493,7 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
         F.words_cap = L.words.bytes / 4;
         F.runs = (run *)L.runs.ptr;
         F.steps = (step *)L.steps.ptr;
-        F.runs_cap = L.runs.bytes / sizeof(run);
+        F.runs_cap = run_capacity(L);
         F.shadow = (float *)L.shadow.ptr;
         F.shadow_cap = L.shadow.bytes / 4;
         F.counters = (unsigned long long *)L.counters.ptr;
@@ -2040,7 +2046,7 @@ int finish_server_batch(cih_server *sv, int k) {
             auto grow = [](uint64_t asked, uint64_t cap) { return asked > cap ? 2 * asked : cap; };
             int s = ensure_arenas(L, grow(c[COUNTER_POINTS], L.points.bytes / 8),
                                   grow(c[COUNTER_WORDS], L.words.bytes / 4),
-                                  grow(c[COUNTER_RUNS], L.runs.bytes / sizeof(run)),
+                                  grow(c[COUNTER_RUNS], run_capacity(L)),
                                   grow(c[COUNTER_SHADOW], L.shadow.bytes / 4));
             if (s)
                 return s;
