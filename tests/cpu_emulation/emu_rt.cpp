@@ -228,10 +228,31 @@ void run_kernel(const char *name, dim3 grid, dim3 blk, void (*body)(void *), voi
     g_gdim = {grid.x, grid.y, grid.z};
     if (blk.x * blk.y * blk.z > 1024)
         fail("block too large");
-    for (unsigned z = 0; z < grid.z; ++z)
-        for (unsigned y = 0; y < grid.y; ++y)
-            for (unsigned x = 0; x < grid.x; ++x)
-                run_block(x, y, z);
+    // EMU_SHUFFLE=<seed>: workgroups in a seeded random order (a GPU runs
+    // them concurrently, in no fixed order), else in index order
+    static const char *shuffle = getenv("EMU_SHUFFLE");
+    if (shuffle && *shuffle) {
+        size_t total = (size_t)grid.x * grid.y * grid.z;
+        std::vector<size_t> order(total);
+        for (size_t i = 0; i < total; ++i)
+            order[i] = i;
+        unsigned long long st = strtoull(shuffle, nullptr, 0) ^ (kernels_run * 0x9E3779B97F4A7C15ull);
+        for (size_t i = total; i > 1; --i) { // Fisher-Yates with SplitMix64
+            unsigned long long z = (st += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            std::swap(order[i - 1], order[(size_t)(z % i)]);
+        }
+        for (size_t b : order)
+            run_block((unsigned)(b % grid.x), (unsigned)((b / grid.x) % grid.y),
+                      (unsigned)(b / ((size_t)grid.x * grid.y)));
+    } else {
+        for (unsigned z = 0; z < grid.z; ++z)
+            for (unsigned y = 0; y < grid.y; ++y)
+                for (unsigned x = 0; x < grid.x; ++x)
+                    run_block(x, y, z);
+    }
     kernels_run++;
 }
 

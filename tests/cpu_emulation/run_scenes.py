@@ -58,16 +58,32 @@ def batch_and_server(device):
         assert np.array_equal(alone[i][0], got[rid]), i
 
 
+def batch_small(device):
+    """Small configurations, a clip scene and a text scene as one multi-canvas
+    batch (one frame): each canvas as rendered alone."""
+    all_streams = streams()
+    names = ["plain_small", "large_small", "effects_small", "clip_nested", "text_scene_1"]
+    alone = [cia.render(all_streams[n], device) for n in names]
+    b = cia.Batch([all_streams[n] for n in names], device)
+    b.execute(to_host=True)
+    for i, n in enumerate(names):
+        assert np.array_equal(alone[i][1].view(np.uint32), b.linear(i).view(np.uint32)), n
+    b.close()
+
+
+MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small}
+
+
 def main():
     cia.LIBRARY = sys.argv[1]
     all_streams = streams()
     device = cia.default_device()
     failed = False
-    if sys.argv[2:] == ["batch_and_server"]:
-        batch_and_server(device)
-        print("batch_and_server ok", flush=True)
-        return 0
     for name in sys.argv[2:]:
+        if name in MODES:
+            MODES[name](device)
+            print(name, "ok", flush=True)
+            continue
         s = all_streams[name]
         g8, gl = cia.render(s, device)
         r8, rl = oracle.render(s, "port")

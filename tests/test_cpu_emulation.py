@@ -34,10 +34,13 @@ def library():
     return emulation.build()
 
 
-def run(library, names):
+def run(library, names, shuffle=None):
     env = dict(os.environ)
     env["LD_PRELOAD"] = emulation.asan_runtime()
     env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=1"
+    env.pop("EMU_SHUFFLE", None)
+    if shuffle is not None:
+        env["EMU_SHUFFLE"] = shuffle
     p = subprocess.run([sys.executable, os.path.join(HERE, "cpu_emulation", "run_scenes.py"),
                         library] + names, env=env, capture_output=True, text=True,
                        timeout=900)
@@ -65,3 +68,11 @@ def test_put_image_data_in_batches_and_server(library):
     # painter's order: the same pixels as the plain canvas, which applies them
     # at once
     run(library, ["batch_and_server"])
+
+
+def test_workgroups_in_random_order(library):
+    # a GPU runs a kernel's workgroups concurrently, in no fixed order; here
+    # they run in a seeded random order (EMU_SHUFFLE), and the renders, alone
+    # and as one multi-canvas batch, must still be memory-safe and in parity
+    run(library, ["large_small", "effects_small", "clip_nested", "text_scene_1", "batch_small"],
+        shuffle="0xC4")
