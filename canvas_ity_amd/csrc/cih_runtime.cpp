@@ -249,6 +249,21 @@ int set_device(cih_device *d) {
     return CIH_OK;
 }
 
+// Host <-> device copies of the library go on the device's stream and are
+// waited for there.  The stream is non-blocking, so a copy on the NULL stream
+// (plain hipMemcpy) is not ordered before the kernels the stream runs next,
+// and a pageable host-to-device hipMemcpy may return before its transfer has
+// landed; kernels could then read half-written draw, canvas or mask tables.
+hipError_t copy_sync(cih_device *d, void *dst, const void *src, size_t bytes,
+                     hipMemcpyKind kind) {
+    if (!bytes)
+        return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, d->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(d->stream);
+    return e;
+}
+
 // Decodes one packet; returns the number of bytes consumed or 0 if malformed.
 size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc &out,
                      const cih_device *dev) {
@@ -557,8 +572,8 @@ int ensure_mask(cih_canvas *cv, uint32_t id) {
             return s;
         std::vector<mask_desc> zero(kMaxMasks);
         std::memset(zero.data(), 0, sizeof(mask_desc) * kMaxMasks);
-        HIP_TRY(hipMemcpy(cv->mask_table.ptr, zero.data(), sizeof(mask_desc) * kMaxMasks,
-                          hipMemcpyHostToDevice));
+        HIP_TRY(copy_sync(cv->device, cv->mask_table.ptr, zero.data(),
+                          sizeof(mask_desc) * kMaxMasks, hipMemcpyHostToDevice));
         cv->mask_store.resize(kMaxMasks);
         cv->mask_defined.assign(kMaxMasks, false);
     }
@@ -580,7 +595,7 @@ int ensure_mask(cih_canvas *cv, uint32_t id) {
     m.counts = (uint64_t)(uintptr_t)(base + 16 * rows * cap);
     m.last_row = -1;
     // only the new entry: the device has written the others' last rows
-    HIP_TRY(hipMemcpy((mask_desc *)cv->mask_table.ptr + id, &m, sizeof(m),
+    HIP_TRY(copy_sync(cv->device, (mask_desc *)cv->mask_table.ptr + id, &m, sizeof(m),
                       hipMemcpyHostToDevice));
     return CIH_OK;
 }
@@ -780,7 +795,8 @@ int cih_device_open(int ordinal, cih_device **device) {
     // serving lanes) hold its address while replay threads add images.
     if ((s = d->lut.reserve(sizeof(table))) ||
         (s = d->image_table.reserve(sizeof(image_desc) * kMaxImages)) ||
-        (e = hipMemcpy(d->lut.ptr, table, sizeof(table), hipMemcpyHostToDevice)) != hipSuccess) {
+        (e = copy_sync(d, d->lut.ptr, table, sizeof(table), hipMemcpyHostToDevice)) !=
+            hipSuccess) {
         d->lut.release();
         d->image_table.release();
         hipStreamDestroy(d->stream);
@@ -1185,7 +1201,8 @@ int cih_trace_stage(cih_canvas *c, int draw, int stage, float *points, size_t po
     if ((size_t)draw >= d->main.last_ndraws)
         return fail(CIH_ERROR_INVALID, "no such draw in the last frame");
     draw_state S;
-    HIP_TRY(hipMemcpy(&S, (draw_state *)d->main.state.ptr + draw, sizeof(S), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(d, &S, (draw_state *)d->main.state.ptr + draw, sizeof(S),
+                      hipMemcpyDeviceToHost));
     uint64_t pts = stage == 0 ? S.flat_points : stage == 1 ? S.dash_points : S.poly_points;
     uint64_t cnt = stage == 0 ? S.flat_counts : stage == 1 ? S.dash_counts : S.poly_counts;
     size_t np = stage == 0 ? S.flat_npoints : stage == 1 ? S.dash_npoints : S.poly_npoints;
@@ -1193,10 +1210,10 @@ int cih_trace_stage(cih_canvas *c, int draw, int stage, float *points, size_t po
     *npoints = np;
     *nsubpaths = ns;
     if (points && point_capacity && np)
-        HIP_TRY(hipMemcpy(points, (const void *)(uintptr_t)pts, 8 * std::min(point_capacity, np),
-                          hipMemcpyDeviceToHost));
+        HIP_TRY(copy_sync(d, points, (const void *)(uintptr_t)pts,
+                          8 * std::min(point_capacity, np), hipMemcpyDeviceToHost));
     if (subpaths && subpath_capacity && ns)
-        HIP_TRY(hipMemcpy(subpaths, (const void *)(uintptr_t)cnt,
+        HIP_TRY(copy_sync(d, subpaths, (const void *)(uintptr_t)cnt,
                           8 * std::min(subpath_capacity, ns), hipMemcpyDeviceToHost));
     return CIH_OK;
 }
@@ -1210,20 +1227,21 @@ int cih_trace_runs(cih_canvas *c, int draw, int pass, int32_t *xs, int32_t *ys, 
     if ((size_t)draw >= d->main.last_ndraws)
         return fail(CIH_ERROR_INVALID, "no such draw in the last frame");
     draw_state S;
-    HIP_TRY(hipMemcpy(&S, (draw_state *)d->main.state.ptr + draw, sizeof(S), hipMemcpyDeviceToHost));
+    HIP_TRY(copy_sync(d, &S, (draw_state *)d->main.state.ptr + draw, sizeof(S),
+                      hipMemcpyDeviceToHost));
     int row0 = pass ? S.srow0 : S.row0, rows = pass ? S.srows : S.rows;
     uint64_t rowoff = pass ? S.srowoff : S.rowoff, base = pass ? S.sruns : S.runs;
     *nruns = 0;
     if (rows <= 0)
         return CIH_OK;
     std::vector<uint32_t> off((size_t)rows + 1);
-    HIP_TRY(hipMemcpy(off.data(), (uint32_t *)d->main.words.ptr + rowoff, 4 * off.size(),
+    HIP_TRY(copy_sync(d, off.data(), (uint32_t *)d->main.words.ptr + rowoff, 4 * off.size(),
                       hipMemcpyDeviceToHost));
     size_t total = off.back();
     *nruns = total;
     std::vector<run> r(total);
     if (total)
-        HIP_TRY(hipMemcpy(r.data(), (run *)d->main.runs.ptr + base, sizeof(run) * total,
+        HIP_TRY(copy_sync(d, r.data(), (run *)d->main.runs.ptr + base, sizeof(run) * total,
                           hipMemcpyDeviceToHost));
     size_t k = 0;
     for (int row = 0; row < rows; ++row)
@@ -1567,19 +1585,21 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
             destroy_batch(b);
             return s;
         }
+        // uploaded on the stream the batch's kernels run on, and waited for
         hipError_t e;
-        if ((e = hipMemcpy(b->payload.ptr, payload.data(), payload.size(),
+        if ((e = copy_sync(d, b->payload.ptr, payload.data(), payload.size(),
                            hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemcpy(b->draws.ptr, b->descs.data(), sizeof(draw_desc) * b->descs.size(),
+            (e = copy_sync(d, b->draws.ptr, b->descs.data(), sizeof(draw_desc) * b->descs.size(),
                            hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemcpy(b->frame_canvases.ptr, frame_canvases.data(),
+            (e = copy_sync(d, b->frame_canvases.ptr, frame_canvases.data(),
                            sizeof(canvas_desc) * frame_canvases.size(),
                            hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemcpy(b->frame_items.ptr, frame_items.data(), 4 * frame_items.size(),
+            (e = copy_sync(d, b->frame_items.ptr, frame_items.data(), 4 * frame_items.size(),
                            hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemcpy(b->all_canvases.ptr, table.data(), sizeof(canvas_desc) * table.size(),
-                           hipMemcpyHostToDevice)) != hipSuccess ||
-            (e = hipMemcpy(b->prefix_dev.ptr, b->pixel_prefix.data(),
+            (e = copy_sync(d, b->all_canvases.ptr, table.data(),
+                           sizeof(canvas_desc) * table.size(), hipMemcpyHostToDevice)) !=
+                hipSuccess ||
+            (e = copy_sync(d, b->prefix_dev.ptr, b->pixel_prefix.data(),
                            8 * b->pixel_prefix.size(), hipMemcpyHostToDevice)) != hipSuccess) {
             destroy_batch(b);
             return hip_fail(e, "batch upload");
