@@ -277,6 +277,69 @@ def nested_squares(evenodd, size=32):
     return s.tobytes()
 
 
+# Fractional windings.  Every coordinate below is a dyadic rational, so each
+# pixel's overlap fraction a with a rectangle is exact in float and the
+# coverage sums are exact too.  Even-odd takes the triangle wave of the
+# winding sum s (period 2): s for s <= 1, 2 - s for 1 < s <= 2.  Omitting the
+# fold (min(fmod(|s|, 2), 1)) or folding with period 1 changes every pixel
+# whose s lies strictly between 1 and 2, and both scenes have such pixels.
+
+DOUBLE_RECT = (3.25, 5.5, 17.375, 9.125)
+
+
+def double_rect(evenodd, size=32, rect=DOUBLE_RECT):
+    """The same fractional rectangle twice in one path: s = 2a per pixel."""
+    s = Stream(size, size)
+    s.set_color(0, 1, 1, 1, 1)
+    s.set_fill_rule(1 if evenodd else 0)
+    s.begin_path()
+    s.rectangle(*rect)
+    s.rectangle(*rect)
+    s.fill()
+    return s.tobytes()
+
+
+RING_OUTER = (2.0, 2.0, 28.0, 28.0)
+RING_INNER = (8.25, 9.5, 12.375, 10.625)
+
+
+def ring(evenodd, size=32):
+    """A pixel-aligned square with a fractional square inside it, both in the
+    same orientation: s = 1 + a_inner per pixel of the outer square."""
+    s = Stream(size, size)
+    s.set_color(0, 1, 1, 1, 1)
+    s.set_fill_rule(1 if evenodd else 0)
+    s.begin_path()
+    s.rectangle(*RING_OUTER)
+    s.rectangle(*RING_INNER)
+    s.fill()
+    return s.tobytes()
+
+
+def even_odd(s):
+    """Even-odd coverage of a winding sum s in [0, 2]."""
+    s = np.abs(np.asarray(s, dtype=np.float64))
+    return np.where(s > 1.0, 2.0 - s, s)
+
+
+def expected_double_rect(evenodd, size=32, rect=DOUBLE_RECT):
+    s = 2.0 * expected_rect_alpha(size, rect)
+    return even_odd(s) if evenodd else np.minimum(s, 1.0)
+
+
+def expected_ring(evenodd, size=32):
+    outer = expected_rect_alpha(size, RING_OUTER)
+    s = outer + expected_rect_alpha(size, RING_INNER)
+    return even_odd(s) if evenodd else np.minimum(s, 1.0)
+
+
+def winding_sums(size=32):
+    """The winding sums of both scenes, for checking that they reach the
+    interval (1, 2) where the fold matters."""
+    return (2.0 * expected_rect_alpha(size, DOUBLE_RECT),
+            expected_rect_alpha(size, RING_OUTER) + expected_rect_alpha(size, RING_INNER))
+
+
 # ---- the reference's exact get_image_data known-answer test --------------------------
 
 def kat_stream():

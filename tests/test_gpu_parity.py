@@ -113,6 +113,18 @@ def test_even_odd_extension(device):
     check(scenes.nested_squares(True), "even-odd vs the oracle's extension")
 
 
+@pytest.mark.parametrize("evenodd", [False, True])
+@pytest.mark.parametrize("name", ["double_rect", "ring"])
+def test_even_odd_fractional_windings(device, name, evenodd):
+    """Even-odd and non-zero coverage at windings in (1, 2) against closed
+    forms (the triangle fold of the winding sum), and against the oracle."""
+    stream = getattr(scenes, name)(evenodd)
+    expected = getattr(scenes, "expected_" + name)(evenodd)
+    _, lin = gpu(stream)
+    assert np.abs(lin[..., 3].astype(np.float64) - expected).max() <= 1e-6
+    check(stream, "%s %s vs the oracle" % (name, "even-odd" if evenodd else "non-zero"))
+
+
 # ---- benchmark configurations --------------------------------------------------------
 
 # ---- clip masks -------------------------------------------------------------------

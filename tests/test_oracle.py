@@ -151,6 +151,32 @@ def test_even_odd_extension():
     assert np.all(eo[5:9, 5:27, 3] == 1.0)
 
 
+@pytest.mark.parametrize("evenodd", [False, True])
+@pytest.mark.parametrize("name", ["double_rect", "ring"])
+def test_even_odd_fractional_windings(name, evenodd):
+    """The oracle's coverage rule against closed forms at windings in (1, 2):
+    two copies of one fractional rectangle (s = 2a) and a square with a
+    fractional inner square (s = 1 + a)."""
+    build = getattr(scenes, name)
+    expected = getattr(scenes, "expected_" + name)(evenodd)
+    _, lin = render(build(evenodd))
+    assert np.abs(lin[..., 3].astype(np.float64) - expected).max() <= 1e-6
+
+
+def test_even_odd_scenes_reach_the_fold():
+    """Both fractional-winding scenes have pixels with 1 < s < 2, where an
+    even-odd rule without the triangle fold, or folded with period 1, gives
+    a different coverage than the closed form."""
+    for s in scenes.winding_sums():
+        fold = (s > 1.0) & (s < 2.0)
+        assert fold.sum() >= 20
+        right = scenes.even_odd(s)[fold]
+        no_fold = np.minimum(np.fmod(s, 2.0), 1.0)[fold]
+        period_one = np.fmod(s, 1.0)[fold]
+        assert np.abs(right - no_fold).min() > 0.01
+        assert np.abs(right - period_one).max() > 0.01
+
+
 def test_malformed_stream_rejected():
     stream = bytearray(scenes.solid())
     stream[16:20] = (99).to_bytes(4, "little")
