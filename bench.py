@@ -30,41 +30,87 @@ METRIC = "Mpix/s + canvases/s (batched vs plain GPU), p99 latency at fixed QPS, 
 SEED = 0xC2
 
 
-def cpu_baseline(stream, width, height, sample_draws, threads):
-    """Times the CPU reference on the first `sample_draws` draws of the stream
-    (single thread, then one instance per thread) and scales to one canvas."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Threads for the one-instance-per-core figure: the CPUs this process may
+    run on, at most 16 (the host share of one GPU on the measurement boxes)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _median_time(fn, runs):
+    fn()  # warm-up
+    times = []
+    for _ in range(runs):
+        t = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t)
+    times.sort()
+    return times[len(times) // 2]
+
+
+def _concurrent_time(fn, threads):
+    """Wall time of `threads` concurrent calls of fn (the renders run in C
+    with the interpreter lock released)."""
+    workers = [threading.Thread(target=fn) for _ in range(threads)]
+    t = time.perf_counter()
+    for w in workers:
+        w.start()
+    for w in workers:
+        w.join()
+    return time.perf_counter() - t
+
+
+def cpu_baseline(cia, stream, width, height, sample_draws, threads, runs=5):
+    """The CPU reference (oracle/_ref, canvas_ity compiled from its source; the
+    C restatement `port` only where that build is absent, and then named so)
+    on this host.  Large-2048 is sampled: one whole canvas takes 40-90 s on
+    one core, so five runs plus the per-core run would not fit the bench's
+    few minutes; the first `sample_draws` draws of the same seeded stream
+    (identically distributed) are timed and scaled to the 5000 draws.
+    Plain-512, the reference's own CPU-sized case, is timed on whole
+    canvases."""
     import oracle
     from canvas_ity_amd.stream import count_draws, truncate_draws
     kind = "reference" if oracle.available("reference") else "port"
+    model = cpu_model()
     total = count_draws(stream)
     sample = truncate_draws(stream, sample_draws)
     scale = total / float(sample_draws)
-    t0 = time.perf_counter()
-    oracle.render(sample, kind)
-    single = time.perf_counter() - t0
-    per_canvas = single * scale
-    result = {"value": width * height / 1e6 / per_canvas, "unit": "Mpix/s", "cores": 1,
-              "kind": kind,
-              "sample": "first %d of %d draws of the same 2048x2048 stream, one thread, "
-                        "scaled by %.3g to a whole canvas (%.1f s measured)"
-                        % (sample_draws, total, scale, single)}
-    if threads > 1:
-        times = [0.0] * threads
-
-        def work(i):
-            t = time.perf_counter()
-            oracle.render(sample, kind)
-            times[i] = time.perf_counter() - t
-        workers = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
-        t0 = time.perf_counter()
-        for w in workers:
-            w.start()
-        for w in workers:
-            w.join()
-        wall = time.perf_counter() - t0
-        result["per_core"] = {"value": threads * width * height / 1e6 / (wall * scale),
-                              "unit": "Mpix/s", "cores": threads,
-                              "sample": "%d concurrent instances of the sample" % threads}
+    single = _median_time(lambda: oracle.render(sample, kind), runs) * scale
+    wall = _concurrent_time(lambda: oracle.render(sample, kind), threads) * scale
+    result = {
+        "value": round(width * height / 1e6 / single, 4), "unit": "Mpix/s", "cores": 1,
+        "kind": kind, "cpu": model,
+        "sample": "large-2048 stream %#x: first %d of %d draws, median of %d runs after a "
+                  "warm-up, one thread, scaled x%.3g to the whole canvas (%.2f s per canvas)"
+                  % (SEED, sample_draws, total, runs, scale, single),
+        "per_core": {"value": round(threads * width * height / 1e6 / wall, 4),
+                     "unit": "Mpix/s", "cores": threads,
+                     "sample": "%d concurrent instances of the same sample, one per thread, "
+                               "wall time scaled x%.3g" % (threads, scale)}}
+    c1 = cia.scene(1, 0xC1)
+    w1, h1 = cia.stream_size(c1)
+    t1 = _median_time(lambda: oracle.render(c1, kind), runs)
+    tw = _concurrent_time(lambda: oracle.render(c1, kind), threads)
+    result["plain_512_whole_canvas"] = {
+        "canvases_per_s": round(1.0 / t1, 3), "mpix_per_s": round(w1 * h1 / 1e6 / t1, 4),
+        "per_core_canvases_per_s": round(threads / tw, 3), "cores": threads,
+        "sample": "plain-512 stream 0xc1, whole canvas through get_image_data, median of %d "
+                  "runs after a warm-up; then %d concurrent instances" % (runs, threads)}
     return result
 
 
@@ -73,14 +119,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-sample", type=int, default=1500,
-                    help="draws of the workload timed on the CPU reference (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=250,
+                    help="draws of the large-2048 stream timed on the CPU reference (0: skip "
+                         "the CPU baseline)")
     ap.add_argument("--cpu-threads", type=int, default=0,
-                    help="instances for the per-core CPU figure (0: min(16, cores))")
+                    help="instances for the per-core CPU figure (0: the CPUs this process "
+                         "may use, at most 16)")
+    ap.add_argument("--end-to-end", type=int, default=3, metavar="K",
+                    help="canvases timed from command stream to sRGB8 in host memory "
+                         "(replay, upload, render, copy back); 0: skip")
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="also measure the batch mix: N canvases rendered as one batch vs "
-                         "one canvas at a time (canvases/s); a batch holds at most 64M "
-                         "pixels (about 600 canvases of this mix)")
+                         "one canvas at a time (canvases/s); the library refuses batches "
+                         "above 8M pixels or 4096 draws (about 20 canvases of this mix)")
     ap.add_argument("--serve", type=float, default=0.0, metavar="QPS",
                     help="also measure the serving loop: Poisson arrivals at QPS requests/s "
                          "of the serving mix, p50/p99 latency (0: skip)")
@@ -95,8 +146,8 @@ def main():
     if args.copy_probe:
         return copy_probe(args.copy_probe)
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build renders independent canvases per GPU "
-              "and has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: canvases are independent and this build runs "
+              "on one GPU; it has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
         return 2
 
     import canvas_ity_amd as cia
@@ -138,7 +189,7 @@ def main():
 
     # Algorithmic bytes of the composite kernel: 32 B (fp32 RGBA read + write)
     # per pixel-draw, counted exactly by the library's statistics pass.
-    stats = canvas_stats(cia, canvas, device)
+    stats = canvas.stats()
     pixel_draws = stats["pixel_draws"]
     line = {
         "metric": METRIC, "value": round(mpix, 3), "unit": "Mpix/s", "n_gpus": 1,
@@ -148,19 +199,25 @@ def main():
         "config": {"workload": "large-2048 plain path: 2048x2048 canvas, 5000 seeded shapes "
                                "(fills/strokes, beziers/arcs/rects, gradients, dashes, joins), "
                                "sRGB8 read-back", "canvas": [width, height], "draws": 5000,
-                   "path": "plain per-canvas GPU"},
+                   "path": "plain per-canvas GPU",
+                   "timed": "clear, every kernel of the pipeline and the sRGB8 conversion, "
+                            "from draw packets resident in HBM"},
         "canvases_per_s": round(1000.0 / ms, 3),
         "kernel_ms": {k: round(v, 4) for k, v in kernels.items()},
     }
     fine_ms = kernels.get("k_fine", 0.0)
     achieved = 32.0 * pixel_draws / (fine_ms / 1000.0) / 1e9 if fine_ms > 0 else 0.0
+    traffic, traffic_source = traffic_bytes()
     line["roofline"] = {"kernel": "k_fine", "dominant_kernel": dominant, "bound": "hbm",
                         "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_bytes(),
-                        "pixel_draws": pixel_draws}
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                        "traffic_source": traffic_source, "pixel_draws": pixel_draws}
+    if args.end_to_end > 0:
+        line["end_to_end"] = bench_end_to_end(cia, device, stream, args.end_to_end)
     if args.cpu_sample > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(stream, width, height, args.cpu_sample, threads)
+        threads = args.cpu_threads or cpu_threads()
+        line["cpu_baseline"] = cpu_baseline(cia, stream, width, height, args.cpu_sample,
+                                            threads)
     canvas.close()
     if args.batch:
         line["batch"] = bench_batch(cia, device, args.batch)
@@ -170,13 +227,39 @@ def main():
     return 0
 
 
+def bench_end_to_end(cia, device, stream, k):
+    """Large-2048 from the command stream in host memory to sRGB8 in host
+    memory, one canvas at a time: the host state machine's replay into draw
+    packets, their upload and decoding, the whole pipeline and the copy back
+    over PCIe (the `value` figure starts with packets resident in HBM)."""
+    width, height = cia.stream_size(stream)
+    times = []
+    for _ in range(k + 1):
+        t = time.perf_counter()
+        c = cia.Canvas(width, height, device)
+        c.replay(stream)
+        c.rgba8()  # flushes, renders and reads back
+        times.append(time.perf_counter() - t)
+        c.close()
+    times = sorted(times[1:])
+    t = times[len(times) // 2]
+    return {"ms_per_canvas": round(1000.0 * t, 3),
+            "mpix_per_s": round(width * height / 1e6 / t, 3),
+            "sample": "median of %d canvases after one warm-up: replay, upload, render, "
+                      "sRGB8 to host" % k}
+
+
 def bench_batch(cia, device, n, seed=0xC4, reps=3):
     """Batch mix (canvases log-uniform 64..1024, 10..400 paths each): one
     batch of n canvases vs the same canvases one at a time; both from draws
     resident in HBM, sRGB8 written to device memory."""
     streams = [cia.scene(4, cia.scene_seed(seed, i)) for i in range(n)]
     pixels = sum(w * h for w, h in map(cia.stream_size, streams))
-    b = cia.Batch(streams, device)
+    try:
+        b = cia.Batch(streams, device)
+    except cia.CihError as e:
+        return {"workload": "batch mix: %d canvases (seed %#x)" % (n, seed), "pixels": pixels,
+                "refused": str(e)}
     b.execute(to_host=False)  # sizes the arenas, captures the graph
     b.execute(to_host=False)
     device.synchronize()
@@ -278,19 +361,20 @@ def copy_probe(mib):
     return 0
 
 
-def canvas_stats(cia, canvas, device):
-    return canvas.stats()
-
-
 def traffic_bytes():
-    """HBM bytes per k_fine launch from the committed rocprofv3 counter
-    summary (profiles/k_fine_large2048_pmc.json), or None."""
-    path = os.path.join(ROOT, "profiles", "k_fine_large2048_pmc.json")
+    """HBM bytes per k_fine launch and where the figure comes from: the
+    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE runs of
+    this workload, scaled by a copy probe), which is not measured in this
+    run; (None, reason) when the summary is absent."""
+    rel = os.path.join("profiles", "k_fine_large2048_pmc.json")
     try:
-        with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
+        with open(os.path.join(ROOT, rel)) as f:
+            data = json.load(f)
     except (OSError, ValueError):
-        return None
+        return None, "no counter summary"
+    return data.get("hbm_bytes_per_launch"), (
+        "%s: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, recorded earlier, "
+        "not in this run" % rel)
 
 
 if __name__ == "__main__":

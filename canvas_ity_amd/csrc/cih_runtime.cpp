@@ -40,11 +40,17 @@ hipError_t launch_clear_batch(const void *canvases, const void *pixel_prefix, in
                               uint64_t total, hipStream_t stream);
 const int kFrameKernels = 8; // reset, geometry, runs, coalesce, shadow raster, blur rows/cols, fine
 
-// Canvas pixels one batch (and one server batch) may hold.  A batch of 4096
-// batch-mix canvases (491M pixels) has faulted the device once for a reason
-// not yet found, while batches up to 55M pixels have run cleanly; larger
-// batches are refused until the cause is known.
-const uint64_t kBatchPixelLimit = (uint64_t)64 << 20;
+// What one batch (and one server batch of several requests) may hold.
+// Multi-canvas frames of up to 2,035 draws and 5.0M pixels (76 canvases) have
+// run cleanly on the MI355X.  Frames of 25,187 draws over 128 canvases
+// (14.4M pixels) and larger batches have faulted the device, and the cause
+// has not been confirmed on the device.  Batches above these bounds are
+// refused with CIH_ERROR_BUDGET; the server runs a request above them as a
+// batch of its own (a single-canvas frame, like the plain path).
+const uint64_t kBatchPixelLimit = (uint64_t)8 << 20;
+const size_t kBatchDrawLimit = 4096;
+// Draws per frame of a batch or server batch (as for a single canvas).
+const size_t kFrameDrawLimit = 8192;
 
 // Distinct images one device holds (the image table is allocated once).
 const size_t kMaxImages = (size_t)1 << 16;
@@ -1453,7 +1459,8 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
             destroy_batch(b);
             return fail(CIH_ERROR_BUDGET, "a batch may hold at most " +
                                               std::to_string(kBatchPixelLimit) +
-                                              " canvas pixels in this build");
+                                              " canvas pixels in this build (it has " +
+                                              std::to_string(total) + ")");
         }
         if ((s = b->bitmaps.reserve(16 * total)) || (s = b->srgb.reserve(4 * total)) ||
             (s = b->prefix_dev.reserve(8 * b->pixel_prefix.size())) ||
@@ -1523,11 +1530,19 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
             std::vector<unsigned char>().swap(v->pending);
             first_draw[(size_t)i + 1] = b->descs.size();
         }
+        if (b->descs.size() > kBatchDrawLimit) {
+            size_t draws = b->descs.size();
+            destroy_batch(b);
+            return fail(CIH_ERROR_BUDGET, "a batch may hold at most " +
+                                              std::to_string(kBatchDrawLimit) +
+                                              " draws in this build (it has " +
+                                              std::to_string(draws) + ")");
+        }
         // each canvas' clip masks (allocated while decoding)
         for (int i = 0; i < count; ++i)
             table[(size_t)i].masks = (uint64_t)(uintptr_t)b->views[(size_t)i]->mask_table.ptr;
         // frames: consecutive draw ranges of bounded size; canvas lists per frame
-        const size_t max_draws = 32768;
+        const size_t max_draws = kFrameDrawLimit;
         const uint64_t shadow_budget = (uint64_t)1 << 30;
         std::vector<canvas_desc> frame_canvases;
         std::vector<uint32_t> frame_items;
@@ -1940,7 +1955,7 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
     std::vector<canvas_desc> frame_canvases;
     std::vector<uint32_t> frame_items;
     size_t draw = 0;
-    const size_t max_draws = 32768;
+    const size_t max_draws = kFrameDrawLimit;
     const uint64_t shadow_budget = (uint64_t)1 << 30; // shadow floats per frame
     std::vector<unsigned char> used(kCutSlots);
     while (draw < b->descs.size()) {
@@ -2149,7 +2164,7 @@ void server_scheduler(cih_server *sv) {
                 server_request *r = sv->prepared.front();
                 uint64_t rp = (uint64_t)r->width * (uint64_t)r->height;
                 if (!b->requests.empty() &&
-                    (draws + r->descs.size() > (1u << 20) || pixels + rp > kBatchPixelLimit))
+                    (draws + r->descs.size() > kBatchDrawLimit || pixels + rp > kBatchPixelLimit))
                     break;
                 if (!sv->allocator.alloc(r->bytes, r->pool_at))
                     break; // wait for memory: admission by HBM footprint

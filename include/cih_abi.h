@@ -254,8 +254,10 @@ CIH_API int cih_text_outline(const unsigned char *font, size_t font_bytes, float
  * it to pinned host memory and waits.  No reference counterpart: the
  * reference renders one canvas per object (canvas_ity.hpp:194). */
 typedef struct cih_batch cih_batch;
-/* A batch holds at most 64M canvas pixels in this build (CIH_ERROR_BUDGET
- * beyond that); a server batch admits requests up to the same total. */
+/* A batch holds at most 8M (8 * 2^20) canvas pixels and 4096 draws in this
+ * build (CIH_ERROR_BUDGET beyond either).  A server batch admits requests up
+ * to the same totals; a single request above them runs as a batch of its
+ * own. */
 CIH_API int cih_batch_create(cih_device *device, int count, const void *const *streams,
                              const size_t *bytes, cih_batch **batch);
 CIH_API int cih_batch_execute(cih_batch *batch, int to_host);

@@ -71,7 +71,23 @@ def batch_small(device):
     b.close()
 
 
-MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small}
+def batch_limits(device):
+    """Batches above the draw or pixel bound are refused before any kernel
+    runs: the 128-canvas batch of the batch mix (14.4M pixels, 25,187 draws),
+    and three 64x64 canvases of 2000 paths each (over 4096 draws)."""
+    big = [cia.scene(4, cia.scene_seed(0xC4, i)) for i in range(128)]
+    many = [cia.scene(4, cia.scene_seed(0x0C, i), 64, 64, 2000) for i in range(3)]
+    for streams, what in ((big, "canvas pixels"), (many, "draws")):
+        try:
+            cia.Batch(streams, device)
+        except cia.CihError as e:
+            assert "budget" in str(e) and what in str(e), e
+        else:
+            raise AssertionError("a batch above the bound was accepted")
+
+
+MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small,
+         "batch_limits": batch_limits}
 
 
 def main():

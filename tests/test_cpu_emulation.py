@@ -76,3 +76,9 @@ def test_workgroups_in_random_order(library):
     # and as one multi-canvas batch, must still be memory-safe and in parity
     run(library, ["large_small", "effects_small", "clip_nested", "text_scene_1", "batch_small"],
         shuffle="0xC4")
+
+
+def test_batches_above_the_bounds_are_refused(library):
+    # the batch regime that faulted the device (many canvases, tens of
+    # thousands of draws in one frame) is refused with CIH_ERROR_BUDGET
+    run(library, ["batch_limits"])
