@@ -8,7 +8,8 @@
 namespace cih {
 
 // One draw as the kernels see it, decoded from a cih_draw_packet by the host.
-// Offsets are in bytes from the start of the frame's payload buffer.
+// Offsets are in bytes from the start of the frame's payload buffer (64-bit:
+// a batch or server batch joins many canvases' packets into one payload).
 struct draw_desc {
     int32_t kind, op, flags, canvas;
     float global_alpha, line_width, miter_limit, dash_offset;
@@ -22,9 +23,9 @@ struct draw_desc {
     int32_t brush;
     float start_x, start_y, end_x, end_y, start_radius, end_radius;
     int32_t image, image_width, image_height, repetition; // clip: image = the new mask id
-    uint32_t points_at, npoints, subpaths_at, nsubpaths;
-    uint32_t dash_at, ndash, colors_at, ncolors, stops_at, nstops;
-    int32_t clip_mask, pad_; // mask the draw is composited under (0: whole canvas)
+    uint64_t points_at, subpaths_at, dash_at, colors_at, stops_at;
+    uint32_t npoints, nsubpaths, ndash, ncolors, nstops;
+    int32_t clip_mask; // mask the draw is composited under (0: whole canvas)
 };
 
 // Results the kernels leave for the later stages of the same frame.

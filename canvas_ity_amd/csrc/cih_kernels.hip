@@ -1985,11 +1985,12 @@ __device__ inline float lane_float(float v, int l) {
 
 // A non-solid draw's descriptor, stops and colours, staged for its pixels.
 struct paint_lds {
-    uint32_t words[sizeof(draw_desc) / 4];
+    alignas(16) uint32_t words[sizeof(draw_desc) / 4]; // read back as a draw_desc
     float stops[STAGE_STOPS];
     float colors[4 * STAGE_STOPS];
 };
 static_assert(sizeof(draw_desc) % 4 == 0, "draw_desc is staged word by word");
+static_assert(alignof(draw_desc) <= 16, "the staged words are aligned for a draw_desc");
 
 // Gradient terms that do not depend on the pixel (hpp:2336-2360), evaluated
 // once per draw with the reference's float operations.

@@ -367,19 +367,19 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
     out.image_width = p.image_width;
     out.image_height = p.image_height;
     out.repetition = p.repetition;
-    uint32_t a = (uint32_t)(at + sizeof(p));
+    uint64_t a = (uint64_t)at + sizeof(p);
     out.points_at = a;
     out.npoints = p.points;
-    a += 8 * p.points;
+    a += 8 * (uint64_t)p.points;
     out.subpaths_at = a;
     out.nsubpaths = p.subpaths;
-    a += 8 * p.subpaths;
+    a += 8 * (uint64_t)p.subpaths;
     out.dash_at = a;
     out.ndash = p.dashes;
-    a += 4 * p.dashes;
+    a += 4 * (uint64_t)p.dashes;
     out.colors_at = a;
     out.ncolors = p.colors;
-    a += 16 * p.colors;
+    a += 16 * (uint64_t)p.colors;
     out.stops_at = a;
     out.nstops = p.stops;
     return p.bytes;
@@ -1942,7 +1942,7 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
         table[i].masks = r->masks ? (uint64_t)(uintptr_t)r->masks->mask_table.ptr : 0;
         table[i].first_draw = 0;
         table[i].ndraws = 0;
-        uint32_t base = (uint32_t)payload.size();
+        uint64_t base = payload.size();
         payload.insert(payload.end(), r->packets.begin(), r->packets.end());
         for (draw_desc dd : r->descs) { // packet offsets move with the concatenation
             dd.points_at += base, dd.subpaths_at += base, dd.dash_at += base;
