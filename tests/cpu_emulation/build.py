@@ -14,6 +14,20 @@ against the oracle without a GPU.
   frame that fits, the frame run again with every arena exactly as large as
   it asked for (canvases restored first), so that a kernel reading past the
   used end of an arena crosses the end of an allocation.
+
+Stricter checks, each switched on by an environment variable of the run:
+
+* EMU_POISON=1      every new device buffer is filled with 0xA5 bytes, and
+                    every __shared__ object at the start of each workgroup
+                    (as a GPU leaves LDS holding an earlier workgroup's data);
+* EMU_REDZONES=1    each arena sub-allocation is followed by a poisoned red
+                    zone of at least 16 bytes, so a draw writing or reading
+                    past its own region aborts even when the next draw's
+                    region follows it;
+* EMU_STRICT_STREAMS=1  a host-to-device hipMemcpy on the NULL stream is
+                    held back (its destination poisoned) until the NULL stream
+                    or the device is synchronised, as the device streams of
+                    the library are non-blocking and do not wait for it.
 """
 import hashlib
 import os
@@ -43,6 +57,54 @@ def _split_top(s):
     if cur.strip():
         parts.append(cur.strip())
     return parts
+
+
+def _declared_names(decl):
+    """Names declared by `TYPE a[..], b;` (without the type)."""
+    names = []
+    for part in _split_top(decl):
+        m = re.search(r"([A-Za-z_][A-Za-z_0-9]*)\s*(\[.*\])?\s*$", part)
+        names.append(m.group(1))
+    return names
+
+
+def _shared_regions(src):
+    """`__shared__ T x;` -> `static T x; emu::shared_region(&x, sizeof(x));`
+    so that EMU_POISON can refill the object at each workgroup's start."""
+    def repl(m):
+        decl = m.group(1)
+        calls = " ".join("emu::shared_region((void *)&%s, sizeof(%s));" % (n, n)
+                         for n in _declared_names(decl))
+        return "static %s; %s" % (decl, calls)
+    return re.sub(r"__shared__\s+([^;]+);", repl, src)
+
+
+ARENA_ALLOC = """__device__ inline uint64_t arena_alloc(const frame &F, int which, uint64_t n, uint64_t cap) {
+    unsigned long long at = atomicAdd(&F.counters[which], (unsigned long long)n);"""
+
+ARENA_ALLOC_REDZONE = """__device__ inline uint64_t arena_alloc(const frame &F, int which, uint64_t n, uint64_t cap) {
+    // emulation: with EMU_REDZONES, a poisoned red zone follows the region
+    uint64_t region = n;
+    if (emu::redzones()) {
+        uint64_t elem = (which == COUNTER_WORDS || which == COUNTER_SHADOW) ? 4 : 8;
+        uint64_t bytes = ((n * elem + 16 + 7) / 8) * 8;
+        n = bytes / elem;
+    }
+    unsigned long long at = atomicAdd(&F.counters[which], (unsigned long long)n);
+    if (emu::redzones() && at + n <= cap && n > region) {
+        uint64_t elem = (which == COUNTER_WORDS || which == COUNTER_SHADOW) ? 4 : 8;
+        char *base = which == COUNTER_POINTS ? (char *)F.points
+                   : which == COUNTER_WORDS  ? (char *)F.words
+                   : which == COUNTER_RUNS   ? (char *)F.runs : (char *)F.shadow;
+        emu::poison(base + (at + region) * elem, (n - region) * elem);
+        if (which == COUNTER_RUNS) // steps share the runs' indices
+            emu::poison((char *)F.steps + (at + region) * elem, (n - region) * elem);
+    }"""
+
+
+def _redzone_arena_alloc(src):
+    assert ARENA_ALLOC in src, "arena_alloc changed: update the emulator's red zones"
+    return src.replace(ARENA_ALLOC, ARENA_ALLOC_REDZONE)
 
 
 def emulated_kernels(src):
@@ -83,7 +145,8 @@ def emulated_kernels(src):
         out += src[i:k] + "EMU_LAUNCH(%s, (%s), (%s)%s)" % (
             name, cfg[0], cfg[1], (", " + args) if args.strip() else "")
         i = p + 1
-    out = out.replace("__shared__", "static")
+    out = _shared_regions(out)
+    out = _redzone_arena_alloc(out)
     out = re.sub(r'asm volatile\("s_waitcnt lgkmcnt\(0\)" ::: "memory"\);',
                  "emu::wave_barrier(__LINE__);", out)
     assert "asm" not in re.sub(r"//.*", "", out), "unhandled inline assembly"
@@ -138,6 +201,15 @@ SAVE_CANVASES = """    L.last_ndraws = n;
 """
 
 
+UNPOISON_ARENAS = """        // emulation: arena red zones of an earlier frame are cleared
+        if (emu::redzones()) {
+            dbuf *arenas[] = {&L.points, &L.words, &L.runs, &L.steps, &L.shadow};
+            for (dbuf *a : arenas)
+                emu::unpoison(a->ptr, a->bytes);
+        }
+        HIP_TRY(launch_frame(F,"""
+
+
 def emulated_runtime(src):
     """cih_runtime.cpp with exact-size buffers and a tight re-run of each
     checked frame."""
@@ -145,6 +217,7 @@ def emulated_runtime(src):
         nonlocal src
         assert old in src, old[:60]
         src = src.replace(old, new)
+    sub("        HIP_TRY(launch_frame(F,", UNPOISON_ARENAS)
     sub("std::max(need, bytes + bytes / 2)", "need")
     sub("attempt < 8;", "attempt < 64;")
     sub("    L.last_ndraws = n;\n", SAVE_CANVASES)

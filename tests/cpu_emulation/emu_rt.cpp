@@ -9,6 +9,7 @@
 #include <ucontext.h>
 #include <vector>
 #include <mutex>
+#include <unordered_map>
 #include <sanitizer/asan_interface.h>
 #include <sanitizer/common_interface_defs.h>
 
@@ -52,6 +53,38 @@ static void *g_ctx = nullptr;
 static const char *g_name = "";
 static unsigned long long progress = 0;
 static thread_local idx3 host_idx = {0, 0, 0};
+static unsigned long long workgroups_started = 0;
+
+static bool env_on(const char *name) {
+    const char *v = getenv(name);
+    return v && *v && *v != '0';
+}
+
+bool redzones() {
+    static const bool on = env_on("EMU_REDZONES");
+    return on;
+}
+
+static bool poison_buffers() {
+    static const bool on = env_on("EMU_POISON");
+    return on;
+}
+
+void poison(void *p, size_t n) { ASAN_POISON_MEMORY_REGION(p, n); }
+void unpoison(void *p, size_t n) { ASAN_UNPOISON_MEMORY_REGION(p, n); }
+
+// A __shared__ object (function-static here) refilled with 0xA5 the first time
+// a lane of each workgroup reaches its declaration.
+void shared_region(void *p, size_t n) {
+    static std::unordered_map<void *, unsigned long long> filled;
+    if (!poison_buffers())
+        return;
+    unsigned long long &at = filled[p];
+    if (at != workgroups_started) {
+        at = workgroups_started;
+        memset(p, 0xA5, n);
+    }
+}
 
 idx3 &thread_idx() { return cur ? cur->tid : host_idx; }
 idx3 &block_idx() { return g_block; }
@@ -149,6 +182,7 @@ static void fiber_entry() {
 
 static void run_block(unsigned bx, unsigned by, unsigned bz) {
     g_block = {bx, by, bz};
+    workgroups_started++;
     unsigned n = g_bdim.x * g_bdim.y * g_bdim.z;
     int nwaves = (int)((n + 63) / 64);
     for (int k = 0; k < nwaves; ++k) {
@@ -263,17 +297,71 @@ struct emu_stream { int id; };
 struct emu_event { int id; };
 static int g_err = hipSuccess;
 
+// EMU_STRICT_STREAMS: host-to-device copies on the NULL stream are held back
+// until the NULL stream or the device is synchronised (or another NULL-stream
+// copy, hipFree or hipHostFree orders after them), with their destination
+// poisoned meanwhile.  The library's streams are non-blocking: work on them
+// does not wait for the NULL stream, so a kernel that reads such a
+// destination first aborts under AddressSanitizer.
+struct held_copy {
+    char *dst;
+    std::vector<char> data;
+};
+static std::mutex held_mutex;
+static std::vector<held_copy> held;
+
+static bool strict_streams() {
+    static const bool on = emu::env_on("EMU_STRICT_STREAMS");
+    return on;
+}
+
+static void land_held_copies() {
+    std::lock_guard<std::mutex> lock(held_mutex);
+    for (held_copy &c : held) {
+        ASAN_UNPOISON_MEMORY_REGION(c.dst, c.data.size());
+        memmove(c.dst, c.data.data(), c.data.size());
+    }
+    held.clear();
+}
+
 hipError_t hipMalloc(void **p, size_t n) {
+    *p = malloc(n ? n : 1);
+    if (*p && emu::poison_buffers())
+        memset(*p, 0xA5, n ? n : 1);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipFree(void *p) { land_held_copies(); free(p); return hipSuccess; }
+hipError_t hipHostMalloc(void **p, size_t n, unsigned) {
     *p = malloc(n ? n : 1);
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
-hipError_t hipFree(void *p) { free(p); return hipSuccess; }
-hipError_t hipHostMalloc(void **p, size_t n, unsigned) { return hipMalloc(p, n); }
-hipError_t hipHostFree(void *p) { free(p); return hipSuccess; }
-hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { if (n) memmove(d, s, n); return hipSuccess; }
-hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t) { return hipMemcpy(d, s, n, k); }
+hipError_t hipHostFree(void *p) { land_held_copies(); free(p); return hipSuccess; }
+hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k) {
+    land_held_copies(); // the NULL stream runs its copies in order
+    if (!n)
+        return hipSuccess;
+    if (strict_streams() && k == hipMemcpyHostToDevice) {
+        std::lock_guard<std::mutex> lock(held_mutex);
+        held.push_back(held_copy{(char *)d, std::vector<char>((const char *)s, (const char *)s + n)});
+        ASAN_POISON_MEMORY_REGION(d, n);
+        return hipSuccess;
+    }
+    memmove(d, s, n);
+    return hipSuccess;
+}
+hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t st) {
+    if (!st)
+        return hipMemcpy(d, s, n, k);
+    if (n)
+        memmove(d, s, n); // stream-ordered: kernels of the stream run later
+    return hipSuccess;
+}
 hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
-hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamSynchronize(hipStream_t st) {
+    if (!st)
+        land_held_copies();
+    return hipSuccess;
+}
 hipError_t hipStreamCreate(hipStream_t *s) { *s = new emu_stream{1}; return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { return hipStreamCreate(s); }
 hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }

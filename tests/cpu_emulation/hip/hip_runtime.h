@@ -77,6 +77,13 @@ void block_barrier(int site);
 void wave_barrier(int site);
 void fail(const char *msg);
 void run_kernel(const char *name, dim3 grid, dim3 block, void (*body)(void *), void *ctx);
+// stricter checks (see tests/cpu_emulation/build.py): EMU_POISON refills each
+// __shared__ object at every workgroup's start; EMU_REDZONES puts a poisoned
+// red zone after each arena sub-allocation
+bool redzones();
+void poison(void *p, size_t n);
+void unpoison(void *p, size_t n);
+void shared_region(void *p, size_t n);
 }
 
 #define threadIdx (emu::thread_idx())

@@ -34,13 +34,19 @@ def library():
     return emulation.build()
 
 
-def run(library, names, shuffle=None):
+STRICT = {"EMU_POISON": "1", "EMU_REDZONES": "1", "EMU_STRICT_STREAMS": "1"}
+
+
+def run(library, names, shuffle=None, strict=False):
     env = dict(os.environ)
     env["LD_PRELOAD"] = emulation.asan_runtime()
     env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=1"
-    env.pop("EMU_SHUFFLE", None)
+    for k in ["EMU_SHUFFLE"] + list(STRICT):
+        env.pop(k, None)
     if shuffle is not None:
         env["EMU_SHUFFLE"] = shuffle
+    if strict:
+        env.update(STRICT)
     p = subprocess.run([sys.executable, os.path.join(HERE, "cpu_emulation", "run_scenes.py"),
                         library] + names, env=env, capture_output=True, text=True,
                        timeout=900)
@@ -82,3 +88,14 @@ def test_batches_above_the_bounds_are_refused(library):
     # the batch regime that faulted the device (many canvases, tens of
     # thousands of draws in one frame) is refused with CIH_ERROR_BUDGET
     run(library, ["batch_limits"])
+
+
+def test_strict_checks(library):
+    # device buffers and LDS start as 0xA5 garbage, every arena sub-allocation
+    # is followed by a poisoned red zone (a draw overrunning into the next
+    # draw's region aborts), and NULL-stream uploads are held back until the
+    # NULL stream is synchronised (the library's streams do not wait for it):
+    # single canvases, a multi-canvas batch with its captured re-run, and the
+    # two-lane server must stay memory-safe and in parity
+    run(library, ["large_small", "effects_small", "clip_nested", "text_scene_1", "batch_small",
+                  "batch_and_server"], strict=True)
