@@ -184,12 +184,11 @@ struct image_entry {
 struct exec_lane {
     hipStream_t stream = nullptr;
     dbuf points, words, runs, steps, shadow, counters; // arenas and their counters
-    dbuf state, canvases, items;                 // frame inputs and per-draw results
+    dbuf state;                                  // per-draw results and the work list
     hbuf host_counters;
     size_t last_ndraws = 0; // draws of the lane's last frame (for the trace seams)
     void release() {
-        dbuf *all[] = {&points, &words, &runs, &steps, &shadow, &counters, &state, &canvases,
-                       &items};
+        dbuf *all[] = {&points, &words, &runs, &steps, &shadow, &counters, &state};
         for (dbuf *b : all)
             b->release();
         host_counters.release();
@@ -218,12 +217,27 @@ struct cih_device {
     size_t events_used = 0;
 };
 
-// A canvas' decoded draw list, resident in HBM so it can be run again.
+// What a frame needs from the arenas, and its fine-stage work items.
+struct frame_shape {
+    bool any_shadow = false;
+    int max_rows = 0, max_cols = 0;
+    uint64_t shadow_bound = 0, in_points = 0;
+    size_t ndraws = 0;
+    std::vector<uint32_t> items; // per-canvas prefix of (row, segment) items
+    std::vector<int32_t> clips;  // (draw, canvas height) of each clip draw, in order
+    std::vector<int32_t> puts;   // (draw, texel count) of each put, in order
+    bool masked = false;         // some draw is composited under a clip mask
+};
+
+// A canvas' decoded draw list, resident in HBM so it can be run again, with
+// each frame's canvas record and fine-stage work items (uploaded once).
 struct program {
     dbuf payload, draws;
+    dbuf canvases, items; // one canvas_desc per frame; the frames' item prefixes
     std::vector<draw_desc> descs; // host copy
     struct span {
-        size_t first, count;
+        size_t first, count, items_at;
+        frame_shape shape;
     };
     std::vector<span> frames;
     size_t bytes = 0;
@@ -418,26 +432,7 @@ int profile_events(cih_device *d, hipEvent_t **events) {
     return CIH_OK;
 }
 
-// A frame: draws (already in HBM) of one or more canvases, run together.
-struct frame_input {
-    const unsigned char *payload; // device
-    const draw_desc *draws;       // device, n records
-    const draw_desc *host;        // host copy of the same records
-    size_t n;
-    std::vector<canvas_desc> canvases; // first_draw relative to `draws`
-};
 
-// What a frame needs from the arenas, and its fine-stage work items.
-struct frame_shape {
-    bool any_shadow = false;
-    int max_rows = 0, max_cols = 0;
-    uint64_t shadow_bound = 0, in_points = 0;
-    size_t ndraws = 0;
-    std::vector<uint32_t> items; // per-canvas prefix of (row, segment) items
-    std::vector<int32_t> clips;  // (draw, canvas height) of each clip draw, in order
-    std::vector<int32_t> puts;   // (draw, texel count) of each put, in order
-    bool masked = false;         // some draw is composited under a clip mask
-};
 
 frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t ncanvases) {
     frame_shape f;
@@ -553,21 +548,6 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
     return fail(CIH_ERROR_OUT_OF_MEMORY, "frame did not fit after repeated arena growth");
 }
 
-// Launches one frame of a single canvas (uploads its small canvas list).
-int launch(cih_device *d, const frame_input &in, bool check) {
-    int s;
-    frame_shape shape = shape_of(in.host, in.canvases.data(), in.canvases.size());
-    if ((s = d->main.canvases.reserve(sizeof(canvas_desc) * in.canvases.size())) ||
-        (s = d->main.items.reserve(4 * shape.items.size())))
-        return s;
-    HIP_TRY(hipMemcpyAsync(d->main.canvases.ptr, in.canvases.data(),
-                           sizeof(canvas_desc) * in.canvases.size(), hipMemcpyHostToDevice,
-                           d->stream));
-    HIP_TRY(hipMemcpyAsync(d->main.items.ptr, shape.items.data(), 4 * shape.items.size(),
-                           hipMemcpyHostToDevice, d->stream));
-    return run_kernels(d, d->main, in.payload, in.draws, in.n, (const canvas_desc *)d->main.canvases.ptr,
-                       in.canvases.size(), (const uint32_t *)d->main.items.ptr, shape, check);
-}
 
 // Device storage for clip mask `id` of a canvas (kept once allocated: the id
 // is reused by later clips, which overwrite the mask when they run).
@@ -676,16 +656,11 @@ int prepare_program(cih_canvas *cv) {
         at += used;
     }
     if ((s = P.payload.reserve(std::max<size_t>(bytes.size(), 16))) ||
-        (s = P.draws.reserve(sizeof(draw_desc) * std::max<size_t>(P.descs.size(), 1))) ||
-        (s = d->staging.reserve(bytes.size() + sizeof(draw_desc) * P.descs.size())))
+        (s = P.draws.reserve(sizeof(draw_desc) * std::max<size_t>(P.descs.size(), 1))))
         return s;
-    // one pinned staging copy, two transfers
-    unsigned char *stage = (unsigned char *)d->staging.ptr;
-    std::memcpy(stage, bytes.data(), bytes.size());
-    std::memcpy(stage + bytes.size(), P.descs.data(), sizeof(draw_desc) * P.descs.size());
-    HIP_TRY(hipMemcpyAsync(P.payload.ptr, stage, bytes.size(), hipMemcpyHostToDevice, d->stream));
-    HIP_TRY(hipMemcpyAsync(P.draws.ptr, stage + bytes.size(), sizeof(draw_desc) * P.descs.size(),
-                           hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(copy_sync(d, P.payload.ptr, bytes.data(), bytes.size(), hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(d, P.draws.ptr, P.descs.data(), sizeof(draw_desc) * P.descs.size(),
+                      hipMemcpyHostToDevice));
     // Frames of bounded size: shadow images are the largest intermediates.
     // A frame also ends where frame_cut says (mask id reuse, puts after
     // painting draws).
@@ -696,7 +671,7 @@ int prepare_program(cih_canvas *cv) {
         size_t last = first;
         uint64_t shadow = 0;
         int current = -1;
-        while (last < P.descs.size() && last - first < 8192) {
+        while (last < P.descs.size() && last - first < kFrameDrawLimit) {
             if (frame_cut(P.descs[last], 0, used, current) && last > first)
                 break;
             uint64_t need = 0;
@@ -709,24 +684,17 @@ int prepare_program(cih_canvas *cv) {
             shadow += need;
             ++last;
         }
-        program::span sp = {first, last - first};
+        program::span sp;
+        sp.first = first;
+        sp.count = last - first;
+        sp.items_at = 0;
         P.frames.push_back(sp);
         first = last;
     }
-    P.bytes = bytes.size();
-    return CIH_OK;
-}
-
-// Runs the canvas' program (all its frames) in order.
-int run_program(cih_canvas *cv, bool check) {
-    program &P = cv->program;
-    int s;
-    for (const program::span &sp : P.frames) {
-        frame_input in;
-        in.payload = (const unsigned char *)P.payload.ptr;
-        in.draws = (const draw_desc *)P.draws.ptr + sp.first;
-        in.host = P.descs.data() + sp.first;
-        in.n = sp.count;
+    // each frame's canvas record and work items, resident with the program
+    std::vector<canvas_desc> records;
+    std::vector<uint32_t> items;
+    for (program::span &sp : P.frames) {
         canvas_desc cd;
         cd.bitmap = (uint64_t)(uintptr_t)cv->pixels;
         cd.width = cv->width;
@@ -734,10 +702,37 @@ int run_program(cih_canvas *cv, bool check) {
         cd.first_draw = 0;
         cd.ndraws = (int32_t)sp.count;
         cd.masks = (uint64_t)(uintptr_t)cv->mask_table.ptr;
-        in.canvases.push_back(cd);
-        if ((s = launch(cv->device, in, check)))
-            return s;
+        sp.shape = shape_of(P.descs.data() + sp.first, &cd, 1);
+        sp.items_at = items.size();
+        items.insert(items.end(), sp.shape.items.begin(), sp.shape.items.end());
+        records.push_back(cd);
     }
+    if ((s = P.canvases.reserve(sizeof(canvas_desc) * std::max<size_t>(records.size(), 1))) ||
+        (s = P.items.reserve(4 * std::max<size_t>(items.size(), 1))))
+        return s;
+    HIP_TRY(copy_sync(d, P.canvases.ptr, records.data(), sizeof(canvas_desc) * records.size(),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(copy_sync(d, P.items.ptr, items.data(), 4 * items.size(), hipMemcpyHostToDevice));
+    P.bytes = bytes.size();
+    return CIH_OK;
+}
+
+// Launches frame k of the canvas' program.
+int run_frame(cih_canvas *cv, size_t k, bool check) {
+    program &P = cv->program;
+    const program::span &sp = P.frames[k];
+    return run_kernels(cv->device, cv->device->main, (const unsigned char *)P.payload.ptr,
+                       (const draw_desc *)P.draws.ptr + sp.first, sp.count,
+                       (const canvas_desc *)P.canvases.ptr + k, 1,
+                       (const uint32_t *)P.items.ptr + sp.items_at, sp.shape, check);
+}
+
+// Runs the canvas' program (all its frames) in order.
+int run_program(cih_canvas *cv, bool check) {
+    int s;
+    for (size_t k = 0; k < cv->program.frames.size(); ++k)
+        if ((s = run_frame(cv, k, check)))
+            return s;
     return CIH_OK;
 }
 
@@ -885,6 +880,8 @@ int cih_canvas_destroy(cih_canvas *c) {
     c->bitmap.release();
     c->program.payload.release();
     c->program.draws.release();
+    c->program.canvases.release();
+    c->program.items.release();
     release_masks(c);
     delete c;
     return CIH_OK;
@@ -1130,21 +1127,8 @@ int cih_canvas_stats(cih_canvas *c, uint64_t *stats) {
     for (int k = 0; k < 4; ++k)
         stats[k] = 0;
     HIP_TRY(launch_clear(c->pixels, (size_t)c->width * c->height, d->stream));
-    for (const program::span &sp : c->program.frames) {
-        frame_input in;
-        in.payload = (const unsigned char *)c->program.payload.ptr;
-        in.draws = (const draw_desc *)c->program.draws.ptr + sp.first;
-        in.host = c->program.descs.data() + sp.first;
-        in.n = sp.count;
-        canvas_desc cd;
-        cd.bitmap = (uint64_t)(uintptr_t)c->pixels;
-        cd.width = c->width;
-        cd.height = c->height;
-        cd.first_draw = 0;
-        cd.ndraws = (int32_t)sp.count;
-        cd.masks = (uint64_t)(uintptr_t)c->mask_table.ptr;
-        in.canvases.push_back(cd);
-        if ((s = launch(d, in, true)))
+    for (size_t k = 0; k < c->program.frames.size(); ++k) {
+        if ((s = run_frame(c, k, true)))
             return s;
         const unsigned long long *hc = (const unsigned long long *)d->main.host_counters.ptr;
         stats[0] += hc[COUNTER_PIXEL_DRAWS];
