@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -108,16 +109,22 @@ struct brush {
 
 struct subpath { size_t count; bool closed; };
 
-// One process-wide device, opened on first use.
+// One process-wide device, opened on first use.  Without a device the
+// failure is reported once on stderr and canvases draw nothing (read back as
+// transparent black), like a canvas nothing was drawn on; the reference never
+// aborts.  Define CANVAS_ITY_HIP_ABORT_WITHOUT_DEVICE to abort instead.
 inline cih_device *shared_device() {
     static std::mutex guard;
     static cih_device *device = 0;
+    static bool reported = false;
     std::lock_guard<std::mutex> lock(guard);
-    if (!device && cih_device_open(0, &device) != CIH_OK) {
-        std::fprintf(stderr, "canvas_ity_hip: no HIP device: %s\n", cih_last_error());
-#ifndef CANVAS_ITY_HIP_NO_ABORT
+    if (!device && !reported && cih_device_open(0, &device) != CIH_OK) {
+        std::fprintf(stderr, "canvas_ity_hip: no HIP device, canvases will draw nothing: %s\n",
+                     cih_last_error());
+#ifdef CANVAS_ITY_HIP_ABORT_WITHOUT_DEVICE
         std::abort();
 #endif
+        reported = true;
         device = 0;
     }
     return device;
@@ -140,9 +147,9 @@ public:
             return; // state-only instance (save stack entries)
         device = detail::shared_device();
         if (device && cih_canvas_create(device, width, height, &handle) != CIH_OK) {
-            std::fprintf(stderr, "canvas_ity_hip: canvas %dx%d: %s\n", width, height,
-                         cih_last_error());
-#ifndef CANVAS_ITY_HIP_NO_ABORT
+            std::fprintf(stderr, "canvas_ity_hip: canvas %dx%d will draw nothing: %s\n", width,
+                         height, cih_last_error());
+#ifdef CANVAS_ITY_HIP_ABORT_WITHOUT_DEVICE
             std::abort();
 #endif
             handle = 0;
@@ -524,8 +531,14 @@ public:
     }
     void get_image_data(unsigned char *image, int width, int height, int stride, int x,
                         int y) {
-        if (!image || !handle)
+        if (!image)
             return;
+        if (!handle) { // no device canvas: transparent black, as an untouched canvas
+            if (width > 0 && height > 0)
+                for (int row = 0; row < height; ++row)
+                    std::memset(image + (std::ptrdiff_t)row * stride, 0, 4 * (size_t)width);
+            return;
+        }
         submit();
         cih_canvas_get_image_data(handle, image, width, height, stride, x, y);
     }

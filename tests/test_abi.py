@@ -96,3 +96,46 @@ def test_stream_writer_round_trip():
     assert words[0] == 0x53484943 and words[2] == 16 and words[3] == 8
     assert words[4] == 17 and np.frombuffer(words[6:10].tobytes(), np.float32)[1] == 0.5
     assert count_ops(s.tobytes(), "fill_rectangle") == 1
+
+
+def test_header_without_a_device_draws_nothing(tmp_path):
+    # A C++ caller of the drop-in header on a machine without a GPU: the
+    # reference never aborts, so the header reports the missing device once
+    # and its canvases read back as transparent black.
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("g++ is not available")
+    probe = cia.library()
+    import ctypes
+    handle = ctypes.c_void_p()
+    if probe.cih_device_open(0, ctypes.byref(handle)) == 0:
+        probe.cih_device_close(handle)
+        pytest.skip("a GPU is present")
+    src = tmp_path / "caller.cpp"
+    src.write_text(r'''
+#include "canvas_ity_hip.hpp"
+#include <cstdio>
+#include <vector>
+int main() {
+    canvas_ity_hip::canvas c(16, 8);
+    c.set_color(canvas_ity_hip::fill_style, 1.0f, 0.5f, 0.0f, 1.0f);
+    c.fill_rectangle(2.0f, 2.0f, 10.0f, 4.0f);
+    std::vector<unsigned char> rgba(16 * 8 * 4, 0xAB);
+    c.get_image_data(&rgba[0], 16, 8, 16 * 4, 0, 0);
+    for (unsigned char v : rgba)
+        if (v)
+            return 3;
+    std::printf("zeros\n");
+    return 0;
+}
+''')
+    exe = tmp_path / "caller"
+    libdir = os.path.join(ROOT, "canvas_ity_amd")
+    subprocess.run(["g++", "-std=c++11", "-ffp-contract=off", "-I" + os.path.join(ROOT, "include"),
+                    str(src), "-o", str(exe), "-L" + libdir, "-lcanvas_ity_hip",
+                    "-Wl,-rpath," + libdir], check=True)
+    p = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert p.stdout.strip() == "zeros"
+    assert "no HIP device" in p.stderr
