@@ -8,7 +8,8 @@ against the oracle without a GPU.
 * Kernels: the launch syntax `k<<<grid, block, shm, stream>>>(args)` becomes a
   call into the emulator (tests/cpu_emulation/emu_rt.cpp), which runs each
   workgroup's lanes as coroutines that meet at every wave intrinsic and
-  barrier; `__shared__` becomes function-static (workgroups run one at a time).
+  barrier; `__shared__` becomes function-static and thread-local (one
+  workgroup at a time per host thread).
 * Runtime: cih_runtime.cpp against a malloc-backed HIP runtime, with device
   buffers allocated at exactly the requested size and, after each checked
   frame that fits, the frame run again with every arena exactly as large as
@@ -27,7 +28,11 @@ Stricter checks, each switched on by an environment variable of the run:
 * EMU_STRICT_STREAMS=1  a host-to-device hipMemcpy on the NULL stream is
                     held back (its destination poisoned) until the NULL stream
                     or the device is synchronised, as the device streams of
-                    the library are non-blocking and do not wait for it.
+                    the library are non-blocking and do not wait for it;
+* EMU_THREADS=N     a kernel's workgroups run on N host threads at the same
+                    time (`__shared__` objects are thread-local, device
+                    atomics are host atomics), so workgroups interleave as
+                    they do on a GPU.
 """
 import hashlib
 import os
@@ -75,7 +80,7 @@ def _shared_regions(src):
         decl = m.group(1)
         calls = " ".join("emu::shared_region((void *)&%s, sizeof(%s));" % (n, n)
                          for n in _declared_names(decl))
-        return "static %s; %s" % (decl, calls)
+        return "static thread_local %s; %s" % (decl, calls)
     return re.sub(r"__shared__\s+([^;]+);", repl, src)
 
 

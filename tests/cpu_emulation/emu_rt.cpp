@@ -9,6 +9,9 @@
 #include <ucontext.h>
 #include <vector>
 #include <mutex>
+#include <atomic>
+#include <condition_variable>
+#include <thread>
 #include <unordered_map>
 #include <sanitizer/asan_interface.h>
 #include <sanitizer/common_interface_defs.h>
@@ -38,22 +41,27 @@ struct rendezvous {
     bool exited[64];
 };
 
+// A workgroup runs on one host thread; with EMU_THREADS=N a kernel's
+// workgroups are spread over N host threads that run at the same time, so all
+// scheduling state is per thread.  The kernel being run (body, arguments,
+// shape) is shared and read-only while its workgroups run.
 static const size_t kStack = 512 * 1024;
-static fiber fibers[1024];
-static fiber *cur = nullptr;
-static jmp_buf sched_jb;
-static void *sched_fake = nullptr;
-static const void *sched_bottom = nullptr;
-static size_t sched_size = 0;
-static rendezvous waves[16];
-static rendezvous block;
-static idx3 g_block, g_bdim, g_gdim;
+static thread_local fiber *fibers = nullptr; // 1024 lanes, allocated per thread
+static thread_local fiber *cur = nullptr;
+static thread_local jmp_buf sched_jb;
+static thread_local void *sched_fake = nullptr;
+static thread_local const void *sched_bottom = nullptr;
+static thread_local size_t sched_size = 0;
+static thread_local rendezvous waves[16];
+static thread_local rendezvous block;
+static thread_local idx3 g_block;
+static idx3 g_bdim, g_gdim;
 static void (*g_body)(void *) = nullptr;
 static void *g_ctx = nullptr;
 static const char *g_name = "";
-static unsigned long long progress = 0;
+static thread_local unsigned long long progress = 0;
 static thread_local idx3 host_idx = {0, 0, 0};
-static unsigned long long workgroups_started = 0;
+static thread_local unsigned long long workgroups_started = 0;
 
 static bool env_on(const char *name) {
     const char *v = getenv(name);
@@ -76,7 +84,7 @@ void unpoison(void *p, size_t n) { ASAN_UNPOISON_MEMORY_REGION(p, n); }
 // A __shared__ object (function-static here) refilled with 0xA5 the first time
 // a lane of each workgroup reaches its declaration.
 void shared_region(void *p, size_t n) {
-    static std::unordered_map<void *, unsigned long long> filled;
+    static thread_local std::unordered_map<void *, unsigned long long> filled;
     if (!poison_buffers())
         return;
     unsigned long long &at = filled[p];
@@ -181,6 +189,8 @@ static void fiber_entry() {
 }
 
 static void run_block(unsigned bx, unsigned by, unsigned bz) {
+    if (!fibers)
+        fibers = new fiber[1024];
     g_block = {bx, by, bz};
     workgroups_started++;
     unsigned n = g_bdim.x * g_bdim.y * g_bdim.z;
@@ -253,6 +263,73 @@ unsigned long long kernels_run = 0;
 
 static std::recursive_mutex run_mutex; // server threads launch kernels too
 
+// The workgroups of the kernel being run, in the order they are handed out.
+static std::vector<size_t> g_order;
+static std::atomic<size_t> g_next{0};
+
+static void run_order_item(size_t b) {
+    run_block((unsigned)(b % g_gdim.x), (unsigned)((b / g_gdim.x) % g_gdim.y),
+              (unsigned)(b / ((size_t)g_gdim.x * g_gdim.y)));
+}
+
+static void drain() {
+    for (;;) {
+        size_t i = g_next.fetch_add(1);
+        if (i >= g_order.size())
+            return;
+        run_order_item(g_order[i]);
+    }
+}
+
+// EMU_THREADS=N: N - 1 pool threads and the launching thread take
+// workgroups from one counter, so workgroups of a kernel run concurrently.
+struct pool {
+    std::mutex m;
+    std::condition_variable go, done;
+    unsigned long long generation = 0;
+    int busy = 0;
+    std::vector<std::thread> threads;
+    explicit pool(int n) {
+        for (int k = 0; k < n; ++k)
+            threads.emplace_back([this] {
+                unsigned long long seen = 0;
+                for (;;) {
+                    {
+                        std::unique_lock<std::mutex> lock(m);
+                        go.wait(lock, [&] { return generation != seen; });
+                        seen = generation;
+                    }
+                    drain();
+                    std::lock_guard<std::mutex> lock(m);
+                    if (--busy == 0)
+                        done.notify_all();
+                }
+            });
+        for (std::thread &t : threads)
+            t.detach();
+    }
+    void run() {
+        {
+            std::lock_guard<std::mutex> lock(m);
+            busy = (int)threads.size();
+            generation++;
+        }
+        go.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lock(m);
+        done.wait(lock, [&] { return busy == 0; });
+    }
+};
+
+static int emu_threads() {
+    static const int n = [] {
+        const char *v = getenv("EMU_THREADS");
+        int k = v ? atoi(v) : 1;
+        return k < 1 ? 1 : k > 64 ? 64 : k;
+    }();
+    return n;
+}
+
 void run_kernel(const char *name, dim3 grid, dim3 blk, void (*body)(void *), void *ctx) {
     std::lock_guard<std::recursive_mutex> lock(run_mutex);
     g_name = name;
@@ -262,30 +339,32 @@ void run_kernel(const char *name, dim3 grid, dim3 blk, void (*body)(void *), voi
     g_gdim = {grid.x, grid.y, grid.z};
     if (blk.x * blk.y * blk.z > 1024)
         fail("block too large");
+    size_t total = (size_t)grid.x * grid.y * grid.z;
+    g_order.resize(total);
+    for (size_t i = 0; i < total; ++i)
+        g_order[i] = i;
     // EMU_SHUFFLE=<seed>: workgroups in a seeded random order (a GPU runs
     // them concurrently, in no fixed order), else in index order
     static const char *shuffle = getenv("EMU_SHUFFLE");
     if (shuffle && *shuffle) {
-        size_t total = (size_t)grid.x * grid.y * grid.z;
-        std::vector<size_t> order(total);
-        for (size_t i = 0; i < total; ++i)
-            order[i] = i;
         unsigned long long st = strtoull(shuffle, nullptr, 0) ^ (kernels_run * 0x9E3779B97F4A7C15ull);
         for (size_t i = total; i > 1; --i) { // Fisher-Yates with SplitMix64
             unsigned long long z = (st += 0x9E3779B97F4A7C15ull);
             z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
             z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
             z ^= z >> 31;
-            std::swap(order[i - 1], order[(size_t)(z % i)]);
+            std::swap(g_order[i - 1], g_order[(size_t)(z % i)]);
         }
-        for (size_t b : order)
-            run_block((unsigned)(b % grid.x), (unsigned)((b / grid.x) % grid.y),
-                      (unsigned)(b / ((size_t)grid.x * grid.y)));
+    }
+    g_next = 0;
+    int n = emu_threads();
+    if (n > 1) {
+        // never destroyed: its detached threads wait on its condition
+        // variable until the process ends
+        static pool *workers = new pool(n - 1);
+        workers->run();
     } else {
-        for (unsigned z = 0; z < grid.z; ++z)
-            for (unsigned y = 0; y < grid.y; ++y)
-                for (unsigned x = 0; x < grid.x; ++x)
-                    run_block(x, y, z);
+        drain();
     }
     kernels_run++;
 }

@@ -86,8 +86,27 @@ def batch_limits(device):
             raise AssertionError("a batch above the bound was accepted")
 
 
+def batch_mix(device, n):
+    """n canvases of the batch mix (sizes log-uniform 64..1024, 10..400 paths)
+    as one batch: each canvas within the parity bar of the oracle."""
+    streams = [cia.scene(4, cia.scene_seed(0xC4, i)) for i in range(n)]
+    b = cia.Batch(streams, device)
+    b.execute(to_host=True)
+    for i, s in enumerate(streams):
+        r8, rl = oracle.render(s, "port")
+        assert_parity(r8, rl, b.rgba8(i), b.linear(i), "batch-mix canvas %d" % i)
+    b.close()
+
+
 MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small,
          "batch_limits": batch_limits}
+
+
+def mode(name):
+    if name.startswith("batch_mix_"):
+        n = int(name[len("batch_mix_"):])
+        return lambda device: batch_mix(device, n)
+    return MODES.get(name)
 
 
 def main():
@@ -96,8 +115,8 @@ def main():
     device = cia.default_device()
     failed = False
     for name in sys.argv[2:]:
-        if name in MODES:
-            MODES[name](device)
+        if mode(name):
+            mode(name)(device)
             print(name, "ok", flush=True)
             continue
         s = all_streams[name]

@@ -37,12 +37,14 @@ def library():
 STRICT = {"EMU_POISON": "1", "EMU_REDZONES": "1", "EMU_STRICT_STREAMS": "1"}
 
 
-def run(library, names, shuffle=None, strict=False):
+def run(library, names, shuffle=None, strict=False, threads=None):
     env = dict(os.environ)
     env["LD_PRELOAD"] = emulation.asan_runtime()
     env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=1"
-    for k in ["EMU_SHUFFLE"] + list(STRICT):
+    for k in ["EMU_SHUFFLE", "EMU_THREADS"] + list(STRICT):
         env.pop(k, None)
+    if threads is not None:
+        env["EMU_THREADS"] = str(threads)
     if shuffle is not None:
         env["EMU_SHUFFLE"] = shuffle
     if strict:
@@ -99,3 +101,13 @@ def test_strict_checks(library):
     # two-lane server must stay memory-safe and in parity
     run(library, ["large_small", "effects_small", "clip_nested", "text_scene_1", "batch_small",
                   "batch_and_server"], strict=True)
+
+
+def test_concurrent_workgroups(library):
+    # a kernel's workgroups on 8 host threads at once (atomics are host
+    # atomics, LDS is per thread), with the strict checks on: renders alone,
+    # a multi-canvas batch and the server must stay memory-safe and in parity
+    # when workgroups interleave as on a GPU.  (Full-size batch-mix batches:
+    # run_scenes.py batch_mix_<n> with the same environment.)
+    run(library, ["large_small", "effects_small", "clip_nested", "text_scene_1", "batch_small",
+                  "batch_and_server"], strict=True, threads=8)
