@@ -223,6 +223,15 @@ def emulated_runtime(src):
         assert old in src, old[:60]
         src = src.replace(old, new)
     sub("        HIP_TRY(launch_frame(F,", UNPOISON_ARENAS)
+    # EMU_BATCH_DRAWS / EMU_BATCH_PIXELS raise the batch bounds in the
+    # emulation only, so batches that are refused on the device can be run
+    # on the CPU
+    sub("const uint64_t kBatchPixelLimit = (uint64_t)8 << 20;",
+        "const uint64_t kBatchPixelLimit = getenv(\"EMU_BATCH_PIXELS\") ? "
+        "strtoull(getenv(\"EMU_BATCH_PIXELS\"), nullptr, 0) : (uint64_t)8 << 20;")
+    sub("const size_t kBatchDrawLimit = 4096;",
+        "const size_t kBatchDrawLimit = getenv(\"EMU_BATCH_DRAWS\") ? "
+        "strtoull(getenv(\"EMU_BATCH_DRAWS\"), nullptr, 0) : 4096;")
     sub("std::max(need, bytes + bytes / 2)", "need")
     sub("attempt < 8;", "attempt < 64;")
     sub("    L.last_ndraws = n;\n", SAVE_CANVASES)
