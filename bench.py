@@ -132,6 +132,10 @@ def main():
                     help="also measure the batch mix: N canvases rendered as one batch vs "
                          "one canvas at a time (canvases/s); the library refuses batches "
                          "above 8M pixels or 4096 draws (about 20 canvases of this mix)")
+    ap.add_argument("--tune", action="store_true",
+                    help="with --batch: pick the batch's fine-stage launch shape with the "
+                         "autotuner (cached per size class in $CIH_TUNE_CACHE or "
+                         "~/.cache/canvas_ity_hip/fine_tuning.json)")
     ap.add_argument("--serve", type=float, default=0.0, metavar="QPS",
                     help="also measure the serving loop: Poisson arrivals at QPS requests/s "
                          "of the serving mix, p50/p99 latency (0: skip)")
@@ -155,7 +159,7 @@ def main():
         device = cia.default_device()
         line = {}
         if args.batch:
-            line["batch"] = bench_batch(cia, device, args.batch)
+            line["batch"] = bench_batch(cia, device, args.batch, tune=args.tune)
         if args.serve:
             line["serving"] = bench_serve(cia, device, args.serve, args.requests)
         print(json.dumps(line))
@@ -249,7 +253,7 @@ def bench_end_to_end(cia, device, stream, k):
                       "sRGB8 to host" % k}
 
 
-def bench_batch(cia, device, n, seed=0xC4, reps=3):
+def bench_batch(cia, device, n, seed=0xC4, reps=3, tune=False):
     """Batch mix (canvases log-uniform 64..1024, 10..400 paths each): one
     batch of n canvases vs the same canvases one at a time; both from draws
     resident in HBM, sRGB8 written to device memory."""
@@ -260,6 +264,14 @@ def bench_batch(cia, device, n, seed=0xC4, reps=3):
     except cia.CihError as e:
         return {"workload": "batch mix: %d canvases (seed %#x)" % (n, seed), "pixels": pixels,
                 "refused": str(e)}
+    fine = {"config": cia.FINE_CONFIG_NAMES[0], "tuned": False}
+    if tune:
+        from canvas_ity_amd import tuning
+        config, key, tuned_now = tuning.configure(b, streams)
+        fine = {"config": cia.FINE_CONFIG_NAMES[config], "tuned": True, "size_class": key,
+                "tuned_in_this_run": tuned_now,
+                "ms_per_config": dict(zip(cia.FINE_CONFIG_NAMES,
+                                          tuning.load().get(key, {}).get("ms", [])))}
     b.execute(to_host=False)  # sizes the arenas, captures the graph
     b.execute(to_host=False)
     device.synchronize()
@@ -295,7 +307,8 @@ def bench_batch(cia, device, n, seed=0xC4, reps=3):
             "pixels": pixels, "batched_canvases_per_s": round(n / batched, 2),
             "plain_canvases_per_s": round(n / plain, 2), "ratio": round(plain / batched, 3),
             "batched_mpix_per_s": round(pixels / batched / 1e6, 2),
-            "batched_with_srgb8_to_host_canvases_per_s": round(n / with_host, 2)}
+            "batched_with_srgb8_to_host_canvases_per_s": round(n / with_host, 2),
+            "fine_stage": fine}
 
 
 def bench_serve(cia, device, qps, nreq, seed=0xC5, distinct=64):

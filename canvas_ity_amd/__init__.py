@@ -19,6 +19,12 @@ ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "device error",
           -4: "malformed input", -5: "over memory budget", -6: "timeout"}
 ERROR_BUDGET = -5
 
+# Fine-stage configurations of a batch (cih_batch_configure): item width in
+# pixels of a canvas row, and one workgroup per item or a persistent grid.
+FINE_CONFIGS = 6
+FINE_CONFIG_NAMES = ["256 px, grid", "256 px, persistent", "128 px, grid",
+                     "128 px, persistent", "64 px, grid", "64 px, persistent"]
+
 # composite_operation values (canvas_ity.hpp:146-149)
 OPS = {"source_in": 1, "source_copy": 2, "source_out": 3, "destination_in": 4,
        "destination_atop": 7, "lighter": 10, "destination_over": 11,
@@ -63,6 +69,9 @@ SIGNATURES = {
     "cih_batch_image": (_c.c_int, [_P, _c.c_int, _P]),
     "cih_batch_linear": (_c.c_int, [_P, _c.c_int, _P]),
     "cih_batch_stats": (_c.c_int, [_P, _c.POINTER(_c.c_uint64)]),
+    "cih_batch_configure": (_c.c_int, [_P, _c.c_int]),
+    "cih_batch_tune": (_c.c_int, [_P, _c.c_int, _c.POINTER(_c.c_double),
+                                  _c.POINTER(_c.c_int)]),
     "cih_batch_destroy": (_c.c_int, [_P]),
     "cih_server_start": (_c.c_int, [_P, _c.c_uint64, _c.c_int, _c.c_int, _c.POINTER(_P)]),
     "cih_server_submit": (_c.c_int, [_P, _P, _c.c_size_t, _c.POINTER(_c.c_uint64)]),
@@ -387,6 +396,19 @@ class Batch:
         out = (_c.c_uint64 * 4)()
         _check(library().cih_batch_stats(self.handle, out), "cih_batch_stats")
         return {"canvases": out[0], "draws": out[1], "frames": out[2], "pixels": out[3]}
+
+    def configure(self, config):
+        """Fine-stage launch shape, 0..FINE_CONFIGS-1 (FINE_CONFIG_NAMES)."""
+        _check(library().cih_batch_configure(self.handle, int(config)), "cih_batch_configure")
+
+    def tune(self, reps=3):
+        """Times every fine-stage configuration on this batch and keeps the
+        fastest; returns (best, milliseconds per execute of each)."""
+        ms = (_c.c_double * FINE_CONFIGS)()
+        best = _c.c_int()
+        _check(library().cih_batch_tune(self.handle, int(reps), ms, _c.byref(best)),
+               "cih_batch_tune")
+        return best.value, list(ms)
 
     def close(self):
         if self.handle:

@@ -8,8 +8,9 @@
 namespace cih {
 
 // One draw as the kernels see it, decoded from a cih_draw_packet by the host.
-// Offsets are in bytes from the start of the frame's payload buffer (64-bit:
-// a batch or server batch joins many canvases' packets into one payload).
+// Offsets are in bytes from the start of the frame's payload buffer, which
+// the host keeps below 4 GiB (CIH_ERROR_BUDGET beyond): 32-bit offsets keep
+// k_fine's per-draw registers down.
 struct draw_desc {
     int32_t kind, op, flags, canvas;
     float global_alpha, line_width, miter_limit, dash_offset;
@@ -23,9 +24,9 @@ struct draw_desc {
     int32_t brush;
     float start_x, start_y, end_x, end_y, start_radius, end_radius;
     int32_t image, image_width, image_height, repetition; // clip: image = the new mask id
-    uint64_t points_at, subpaths_at, dash_at, colors_at, stops_at;
-    uint32_t npoints, nsubpaths, ndash, ncolors, nstops;
-    int32_t clip_mask; // mask the draw is composited under (0: whole canvas)
+    uint32_t points_at, npoints, subpaths_at, nsubpaths;
+    uint32_t dash_at, ndash, colors_at, ncolors, stops_at, nstops;
+    int32_t clip_mask, pad_; // mask the draw is composited under (0: whole canvas)
 };
 
 // Results the kernels leave for the later stages of the same frame.
@@ -98,7 +99,8 @@ enum counter {
     COUNTER_OVERFLOW = 4,
     COUNTER_QUEUE = 5, // coalesce work list: (entries << 40) + rows
     COUNTER_PIXEL_DRAWS = 6, // pixels composited by k_fine (main and shadow layers)
-    COUNTER_COUNT = 8
+    COUNTER_FINE_QUEUE = 8,  // 8 cursors of the persistent fine kernel's item ranges
+    COUNTER_COUNT = 16
 };
 
 struct frame {
@@ -122,10 +124,11 @@ struct frame {
     // coalesce work list: one entry per (draw, pass) with rows, in the order
     // k_runs appended them: (first row in the list << 20) | (2 * draw + pass)
     unsigned long long *queue;
-    // fine-stage work list: (canvas, row, segment) items
+    // fine-stage work list: (canvas, row, segment) items, per canvas prefix
     const uint32_t *items;
     uint32_t nitems;
-    int32_t segment; // pixels per fine work item
+    int32_t segment;         // pixels per fine work item: 64, 128 or 256
+    int32_t fine_persistent; // k_fine as a fixed grid draining item queues
 };
 
 } // namespace cih

@@ -1495,10 +1495,12 @@ __global__ __launch_bounds__(64) void k_clip_finish(frame F, int d) {
 // Reading steps.  A fine workgroup is one wave owning FINE_SEGMENT pixels
 // [x0, x0 + FINE_SEGMENT) of a row; lane t owns FINE_PX consecutive pixels.
 
-struct seg_lds {
-    float sum[FINE_SEGMENT];
-    uint32_t mask[FINE_SEGMENT / 32]; // pixels with a step
+template <int PX> // pixels per lane: the segment is 64 * PX pixels
+struct seg_lds_t {
+    float sum[64 * PX];
+    uint32_t mask[2 * PX]; // pixels with a step
 };
+typedef seg_lds_t<FINE_PX> seg_lds;
 
 __device__ inline float coverage(float sum, bool evenodd) {
     if (evenodd) {
@@ -1509,7 +1511,8 @@ __device__ inline float coverage(float sum, bool evenodd) {
     return smin(fabsf(sum), 1.0f);
 }
 
-__device__ inline void scatter_step(seg_lds &L, step s, int x0) {
+template <int PX>
+__device__ inline void scatter_step(seg_lds_t<PX> &L, step s, int x0) {
     int p = s.x - x0;
     L.sum[p] = s.sum;
     atomicOr(&L.mask[p >> 5], 1u << (p & 31));
@@ -1524,14 +1527,16 @@ __device__ inline void scatter_step(seg_lds &L, step s, int x0) {
 // segment_window is the second half: every step before index lo has x < x0,
 // every step from index hi on has x >= x0 (hi - lo <= 64), and lane t holds
 // st[lo + t] in s when lo + t < hi.
+template <int PX>
 __device__ bool segment_window(const step *st, uint32_t n, uint32_t lo, uint32_t hi, step s,
-                               int x0, seg_lds &L, float &before, float sums[FINE_PX]);
+                               int x0, seg_lds_t<PX> &L, float &before, float sums[PX]);
 
 // Scatters the steps st[p, n), 64 at a time, for as long as whole chunks lie
 // inside the segment; returns whether any was inside.
-__device__ bool scatter_rest(const step *st, uint32_t n, uint32_t p, int x0, seg_lds &L) {
+template <int PX>
+__device__ bool scatter_rest(const step *st, uint32_t n, uint32_t p, int x0, seg_lds_t<PX> &L) {
     int lane = threadIdx.x & 63;
-    int x1 = x0 + FINE_SEGMENT;
+    int x1 = x0 + (64 * PX);
     bool any = false, go = p < n;
     while (go) {
         bool v = p + (uint32_t)lane < n;
@@ -1551,40 +1556,42 @@ __device__ bool scatter_rest(const step *st, uint32_t n, uint32_t p, int x0, seg
 
 // Each pixel's sum is that of the last scattered step at or left of it, or
 // `before`; leaves L.mask zero.
-__device__ void segment_lookup(seg_lds &L, float before, float sums[FINE_PX]) {
+template <int PX>
+__device__ void segment_lookup(seg_lds_t<PX> &L, float before, float sums[PX]) {
     int lane = threadIdx.x & 63;
     wave_lds_sync();
-    uint32_t mw[FINE_SEGMENT / 32];
+    uint32_t mw[(64 * PX) / 32];
 #pragma unroll
-    for (int k = 0; k < FINE_SEGMENT / 32; ++k)
+    for (int k = 0; k < (64 * PX) / 32; ++k)
         mw[k] = L.mask[k];
-    int base = lane * FINE_PX, wi = base >> 5;
+    int base = lane * PX, wi = base >> 5;
     int prev = -1;
     uint32_t mine = 0;
 #pragma unroll
-    for (int k = 0; k < FINE_SEGMENT / 32; ++k) {
+    for (int k = 0; k < (64 * PX) / 32; ++k) {
         if (k < wi && mw[k])
             prev = k * 32 + 31 - __builtin_clz(mw[k]);
         if (k == wi)
             mine = mw[k];
     }
 #pragma unroll
-    for (int k = 0; k < FINE_PX; ++k) {
+    for (int k = 0; k < PX; ++k) {
         int bit = (base + k) & 31;
         uint32_t mk = mine & (0xffffffffu >> (31 - bit));
         int pos = mk ? wi * 32 + 31 - __builtin_clz(mk) : prev;
         sums[k] = pos >= 0 ? L.sum[pos] : before;
     }
     wave_lds_sync();
-    if (lane < FINE_SEGMENT / 32)
+    if (lane < (64 * PX) / 32)
         L.mask[lane] = 0;
     wave_lds_sync();
 }
 
+template <int PX>
 __device__ bool segment_window(const step *st, uint32_t n, uint32_t lo, uint32_t hi, step s,
-                               int x0, seg_lds &L, float &before, float sums[FINE_PX]) {
+                               int x0, seg_lds_t<PX> &L, float &before, float sums[PX]) {
     int lane = threadIdx.x & 63;
-    int x1 = x0 + FINE_SEGMENT;
+    int x1 = x0 + (64 * PX);
     bool valid = lo + (uint32_t)lane < hi;
     bool lt = valid && s.x < x0;
     int c = __popcll(__ballot(lt));
@@ -1607,9 +1614,10 @@ __device__ bool segment_window(const step *st, uint32_t n, uint32_t lo, uint32_t
 // Like segment_window when the first step at or right of x0 is known: index
 // `first`, with lane t holding st[first + t] (a sentinel past n), and the sum
 // before the segment given.
+template <int PX>
 __device__ bool segment_fill(const step *st, uint32_t n, uint32_t first, step s, int x0,
-                             seg_lds &L, float before, float sums[FINE_PX]) {
-    int x1 = x0 + FINE_SEGMENT;
+                             seg_lds_t<PX> &L, float before, float sums[PX]) {
+    int x1 = x0 + (64 * PX);
     bool in = s.x < x1;
     if (in)
         scatter_step(L, s, x0);
@@ -1623,8 +1631,9 @@ __device__ bool segment_fill(const step *st, uint32_t n, uint32_t first, step s,
     return true;
 }
 
-__device__ bool segment_sums(const step *st, uint32_t n, int x0, seg_lds &L, float &before,
-                             float sums[FINE_PX]) {
+template <int PX>
+__device__ bool segment_sums(const step *st, uint32_t n, int x0, seg_lds_t<PX> &L, float &before,
+                             float sums[PX]) {
     int lane = threadIdx.x & 63;
     // first step with x >= x0: 64-way search down to one wave-wide window
     uint32_t lo = 0, hi = n;
@@ -1963,6 +1972,7 @@ __device__ inline uint32_t find_canvas(const uint32_t *prefix, uint32_t n, uint3
 // b / FINE_XCDS keeps the segments of one row, which read the same steps, in
 // one L2.
 #define FINE_XCDS 8
+#define FINE_PERSISTENT_GRID (256 * 16) // persistent fine waves: 16 per CU
 
 __device__ inline bool fine_item(const frame &F, uint32_t &item) {
     uint32_t per = (F.nitems + FINE_XCDS - 1) / FINE_XCDS;
@@ -1985,12 +1995,12 @@ __device__ inline float lane_float(float v, int l) {
 
 // A non-solid draw's descriptor, stops and colours, staged for its pixels.
 struct paint_lds {
-    alignas(16) uint32_t words[sizeof(draw_desc) / 4]; // read back as a draw_desc
+    uint32_t words[sizeof(draw_desc) / 4]; // read back as a draw_desc
     float stops[STAGE_STOPS];
     float colors[4 * STAGE_STOPS];
 };
 static_assert(sizeof(draw_desc) % 4 == 0, "draw_desc is staged word by word");
-static_assert(alignof(draw_desc) <= 16, "the staged words are aligned for a draw_desc");
+static_assert(alignof(draw_desc) == 4, "the staged words are aligned for a draw_desc");
 
 // Gradient terms that do not depend on the pixel (hpp:2336-2360), evaluated
 // once per draw with the reference's float operations.
@@ -2069,17 +2079,17 @@ __device__ __forceinline__ float4 gradient_pixel(const gradient_setup &G, const 
 // everything).
 enum { FILL_SOLID, FILL_EMPTY, FILL_GRADIENT };
 
-template <int KIND, bool MASKED>
+template <int KIND, bool MASKED, int PX>
 __device__ __forceinline__ void composite_pixels(float4 solid, const gradient_setup &G,
                                                  const float *stops, const float *colors,
-                                                 float4 (&back)[FINE_PX],
-                                                 const int (&xs)[FINE_PX], int width, float py,
-                                                 const float (&sums)[FINE_PX], bool evenodd,
+                                                 float4 (&back)[PX],
+                                                 const int (&xs)[PX], int width, float py,
+                                                 const float (&sums)[PX], bool evenodd,
                                                  bool clear, float galpha, int op,
-                                                 const float (&vis)[FINE_PX],
+                                                 const float (&vis)[PX],
                                                  unsigned &composites) {
 #pragma clang loop unroll(full)
-    for (int k = 0; k < FINE_PX; ++k) {
+    for (int k = 0; k < PX; ++k) {
         float cov = coverage(sums[k], evenodd);
         bool on = xs[k] < width && (cov >= kThreshold || clear);
         if (MASKED)
@@ -2097,31 +2107,29 @@ __device__ __forceinline__ void composite_pixels(float4 solid, const gradient_se
     }
 }
 
-// MASKED: the frame has draws under a clip mask (their visibility is looked
-// up per pixel like coverage, hpp:2565-2604); the plain instantiation keeps
-// the registers of the common case.
-template <bool MASKED>
-__global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
-    __shared__ seg_lds L;
-    __shared__ paint_lds P;
-    uint32_t item;
-    if (!fine_item(F, item) || overflowed(F))
-        return;
+// One fine work item: the segment of PX * 64 pixels of one canvas row, every
+// draw of the canvas applied in painter's order.  MASKED: the frame has draws
+// under a clip mask (their visibility is looked up per pixel like coverage,
+// hpp:2565-2604); the plain instantiation keeps the registers of the common
+// case.  Clears L.mask before its first use.
+template <bool MASKED, int PX>
+__device__ __forceinline__ void fine_item_run(const frame &F, uint32_t item, seg_lds_t<PX> &L,
+                                              paint_lds &P) {
     uint32_t c = find_canvas(F.items, (uint32_t)F.ncanvases, item);
     const canvas_desc &C = F.canvases[c];
     uint32_t local = item - F.items[c];
-    int segs = (C.width + FINE_SEGMENT - 1) / FINE_SEGMENT;
+    int segs = (C.width + (64 * PX) - 1) / (64 * PX);
     int y = (int)(local / (uint32_t)segs);
-    int x0 = (int)(local % (uint32_t)segs) * FINE_SEGMENT;
-    int x1 = x0 + FINE_SEGMENT;
+    int x0 = (int)(local % (uint32_t)segs) * (64 * PX);
+    int x1 = x0 + (64 * PX);
     int lane = threadIdx.x;
     float4 *bitmap = (float4 *)(uintptr_t)C.bitmap + (size_t)y * C.width;
-    if (lane < FINE_SEGMENT / 32)
+    if (lane < 2 * PX)
         L.mask[lane] = 0;
-    float4 back[FINE_PX];
-    int xs[FINE_PX];
-    for (int k = 0; k < FINE_PX; ++k) {
-        xs[k] = x0 + lane * FINE_PX + k;
+    float4 back[PX];
+    int xs[PX];
+    for (int k = 0; k < PX; ++k) {
+        xs[k] = x0 + lane * PX + k;
         back[k] = xs[k] < C.width ? bitmap[xs[k]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     __syncthreads();
@@ -2209,22 +2217,22 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
             int dop = lane_int(op, j);
             // visibility under the draw's clip mask: min(|running sum of the
             // mask's pieces at or left of the pixel|, 1)
-            float vis[FINE_PX];
-            for (int k = 0; k < FINE_PX; ++k)
+            float vis[PX];
+            for (int k = 0; k < PX; ++k)
                 vis[k] = 1.0f;
             if (MASKED) {
                 int m = lane_int(cmask, j);
                 if (m) {
                     const mask_desc &M = ((const mask_desc *)(uintptr_t)C.masks)[m];
                     uint32_t mn = ((const uint32_t *)(uintptr_t)M.counts)[y];
-                    float vb = 0.0f, vsums[FINE_PX];
+                    float vb = 0.0f, vsums[PX];
                     bool vin = false;
                     if (mn)
                         vin = segment_sums((const step *)(uintptr_t)M.steps +
                                                (size_t)y * (size_t)(C.width + 2),
                                            mn, x0, L, vb, vsums);
                     bool any = false;
-                    for (int k = 0; k < FINE_PX; ++k) {
+                    for (int k = 0; k < PX; ++k) {
                         vis[k] = smin(fabsf(vin ? vsums[k] : vb), 1.0f);
                         any = any || vis[k] >= kThreshold;
                     }
@@ -2240,7 +2248,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
                 int lo = max(0, S.left - D.border);
                 int hi = min(C.width, S.left + S.width - D.border);
                 float4 scolor = load4(D.shadow_color);
-                for (int k = 0; k < FINE_PX; ++k) {
+                for (int k = 0; k < PX; ++k) {
                     int x = xs[k];
                     if (x < lo || x >= hi)
                         continue;
@@ -2256,7 +2264,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
                 continue;
             bool clear = !(dop & 8);
             bool evenodd = (lane_int(flags, j) & CIH_DRAW_FLAG_EVENODD) != 0;
-            float sums[FINE_PX];
+            float sums[PX];
             float bj = lane_float(before, j);
             bool inside = false;
             if ((stepped >> j) & 1) {
@@ -2269,7 +2277,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
             if (!inside) {
                 if (!clear && !(coverage(bj, evenodd) >= kThreshold))
                     continue;
-                for (int k = 0; k < FINE_PX; ++k)
+                for (int k = 0; k < PX; ++k)
                     sums[k] = bj;
             }
             float galpha = lane_float(alpha, j);
@@ -2322,7 +2330,7 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
             }
             // patterns: bicubic taps, only for the pixels that are painted
 #pragma clang loop unroll(full)
-            for (int k = 0; k < FINE_PX; ++k) {
+            for (int k = 0; k < PX; ++k) {
                 float cov = coverage(sums[k], evenodd);
                 if (xs[k] < C.width && (cov >= kThreshold || clear) &&
                     (!MASKED || vis[k] >= kThreshold)) {
@@ -2338,9 +2346,49 @@ __global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
         total += __shfl_xor(total, w, 64);
     if (lane == 0 && total)
         atomicAdd(&F.counters[COUNTER_PIXEL_DRAWS], total);
-    for (int k = 0; k < FINE_PX; ++k)
+    for (int k = 0; k < PX; ++k)
         if (xs[k] < C.width)
             bitmap[xs[k]] = back[k];
+}
+
+
+// The fine kernel.  One workgroup (one wave) per work item, or, PERSISTENT,
+// a fixed grid whose waves take items from eight queues until all are empty:
+// the items are split into eight contiguous ranges, one per XCD (a workgroup
+// runs on XCD blockIdx.x % 8), so the segments of a row that read the same
+// steps share an L2; a wave whose own range is empty takes items from the
+// next ranges.  Each range's cursor is a frame counter cleared by k_reset.
+template <bool MASKED, int PX, bool PERSISTENT>
+__global__ __launch_bounds__(FINE_THREADS) void k_fine(frame F) {
+    __shared__ seg_lds_t<PX> L;
+    __shared__ paint_lds P;
+    if (!PERSISTENT) {
+        uint32_t item;
+        if (!fine_item(F, item) || overflowed(F))
+            return;
+        fine_item_run<MASKED, PX>(F, item, L, P);
+        return;
+    }
+    if (overflowed(F))
+        return;
+    uint32_t per = (F.nitems + FINE_XCDS - 1) / FINE_XCDS;
+    int xcd = (int)(blockIdx.x % FINE_XCDS);
+    for (int k = 0; k < FINE_XCDS; ++k) {
+        int q = (xcd + k) % FINE_XCDS;
+        uint32_t lo = (uint32_t)q * per;
+        uint32_t hi = lo + per < F.nitems ? lo + per : F.nitems;
+        if (lo >= hi)
+            continue;
+        for (;;) {
+            uint32_t i = 0;
+            if (threadIdx.x == 0)
+                i = (uint32_t)atomicAdd(&F.counters[COUNTER_FINE_QUEUE + q], 1ull);
+            i = __shfl(i, 0, 64);
+            if (i >= hi - lo)
+                break;
+            fine_item_run<MASKED, PX>(F, lo + i, L, P);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2427,7 +2475,6 @@ __global__ void k_readback(const float4 *bitmap, int W, int H, uint32_t *out, in
 
 namespace cih {
 
-int fine_segment() { return FINE_SEGMENT; }
 
 static unsigned grid_for(size_t n, unsigned block) {
     size_t g = (n + block - 1) / block;
@@ -2508,11 +2555,26 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
         k_blur_cols<<<dim3(F.ndraws, cx > 0 ? cx : 1), 64, 0, stream>>>(F);
     MARK(7);
     if (F.nitems) {
+        // frames with masked draws always take the default 256-pixel item
         unsigned grid = FINE_XCDS * ((F.nitems + FINE_XCDS - 1) / FINE_XCDS);
+        if (F.fine_persistent && !masked && grid > FINE_PERSISTENT_GRID)
+            grid = FINE_PERSISTENT_GRID;
+        int px = masked ? FINE_PX : F.segment / FINE_THREADS;
+        bool persistent = F.fine_persistent && !masked;
         if (masked)
-            k_fine<true><<<grid, FINE_THREADS, 0, stream>>>(F);
+            k_fine<true, FINE_PX, false><<<grid, FINE_THREADS, 0, stream>>>(F);
+        else if (px == 1 && persistent)
+            k_fine<false, 1, true><<<grid, FINE_THREADS, 0, stream>>>(F);
+        else if (px == 1)
+            k_fine<false, 1, false><<<grid, FINE_THREADS, 0, stream>>>(F);
+        else if (px == 2 && persistent)
+            k_fine<false, 2, true><<<grid, FINE_THREADS, 0, stream>>>(F);
+        else if (px == 2)
+            k_fine<false, 2, false><<<grid, FINE_THREADS, 0, stream>>>(F);
+        else if (persistent)
+            k_fine<false, 4, true><<<grid, FINE_THREADS, 0, stream>>>(F);
         else
-            k_fine<false><<<grid, FINE_THREADS, 0, stream>>>(F);
+            k_fine<false, 4, false><<<grid, FINE_THREADS, 0, stream>>>(F);
     }
     MARK(8);
 #undef MARK

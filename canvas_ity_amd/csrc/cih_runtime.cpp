@@ -33,7 +33,6 @@
 #include <vector>
 
 namespace cih {
-int fine_segment();
 hipError_t launch_readback_batch(const void *canvases, const void *pixel_prefix, int ncanvases,
                                  uint64_t total, void *out, hipStream_t stream);
 hipError_t launch_clear_batch(const void *canvases, const void *pixel_prefix, int ncanvases,
@@ -51,6 +50,10 @@ const uint64_t kBatchPixelLimit = (uint64_t)8 << 20;
 const size_t kBatchDrawLimit = 4096;
 // Draws per frame of a batch or server batch (as for a single canvas).
 const size_t kFrameDrawLimit = 8192;
+
+// Bytes of draw packets one canvas program, batch or server batch may hold:
+// draw records address the payload with 32-bit offsets.
+const uint64_t kPayloadLimit = 0xffffffffull;
 
 // Distinct images one device holds (the image table is allocated once).
 const size_t kMaxImages = (size_t)1 << 16;
@@ -217,13 +220,27 @@ struct cih_device {
     size_t events_used = 0;
 };
 
+// Fine-stage item widths: every frame carries the (canvas, row, segment) item
+// prefix for each, so a batch can switch between them (the autotuner does).
+const int kFineVariants = 3;
+const int kFineSegments[kFineVariants] = {256, 128, 64};
+
+// How a frame's fine stage runs: the segment width of a work item (index into
+// kFineSegments) and whether k_fine is a persistent grid draining queues.
+struct fine_config {
+    int variant = 0;
+    bool persistent = false;
+};
+
 // What a frame needs from the arenas, and its fine-stage work items.
 struct frame_shape {
     bool any_shadow = false;
     int max_rows = 0, max_cols = 0;
     uint64_t shadow_bound = 0, in_points = 0;
     size_t ndraws = 0;
-    std::vector<uint32_t> items; // per-canvas prefix of (row, segment) items
+    // per-canvas prefix of (row, segment) items, one array of ncanvases + 1
+    // entries per item width of kFineSegments, in that order
+    std::vector<uint32_t> items;
     std::vector<int32_t> clips;  // (draw, canvas height) of each clip draw, in order
     std::vector<int32_t> puts;   // (draw, texel count) of each put, in order
     bool masked = false;         // some draw is composited under a clip mask
@@ -381,19 +398,19 @@ size_t decode_packet(const unsigned char *base, size_t at, size_t end, draw_desc
     out.image_width = p.image_width;
     out.image_height = p.image_height;
     out.repetition = p.repetition;
-    uint64_t a = (uint64_t)at + sizeof(p);
+    uint32_t a = (uint32_t)(at + sizeof(p));
     out.points_at = a;
     out.npoints = p.points;
-    a += 8 * (uint64_t)p.points;
+    a += 8 * p.points;
     out.subpaths_at = a;
     out.nsubpaths = p.subpaths;
-    a += 8 * (uint64_t)p.subpaths;
+    a += 8 * p.subpaths;
     out.dash_at = a;
     out.ndash = p.dashes;
-    a += 4 * (uint64_t)p.dashes;
+    a += 4 * p.dashes;
     out.colors_at = a;
     out.ncolors = p.colors;
-    a += 16 * (uint64_t)p.colors;
+    a += 16 * p.colors;
     out.stops_at = a;
     out.nstops = p.stops;
     return p.bytes;
@@ -436,12 +453,15 @@ int profile_events(cih_device *d, hipEvent_t **events) {
 
 frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t ncanvases) {
     frame_shape f;
-    int seg = fine_segment();
-    f.items.assign(ncanvases + 1, 0);
+    size_t n1 = ncanvases + 1;
+    f.items.assign(kFineVariants * n1, 0);
     for (size_t c = 0; c < ncanvases; ++c) {
         const canvas_desc &cd = canvases[c];
-        uint64_t segs = (uint64_t)((cd.width + seg - 1) / seg);
-        f.items[c + 1] = f.items[c] + (uint32_t)(segs * (uint64_t)cd.height);
+        for (int v = 0; v < kFineVariants; ++v) {
+            int seg = kFineSegments[v];
+            uint64_t segs = (uint64_t)((cd.width + seg - 1) / seg);
+            f.items[v * n1 + c + 1] = f.items[v * n1 + c] + (uint32_t)(segs * (uint64_t)cd.height);
+        }
         for (int k = cd.first_draw; k < cd.first_draw + cd.ndraws; ++k) {
             const draw_desc &dd = host[k];
             f.in_points += dd.npoints;
@@ -469,6 +489,7 @@ frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t 
 }
 
 // Launches the kernels of one frame whose canvas list and work items are
+// Launches the kernels of one frame whose canvas list and work items are
 // already in HBM.  With `check`, waits for it and, if an arena ran out, grows
 // the arenas to what the frame asked for and runs it again (kernels write no
 // pixel once an arena has overflowed).  Without it nothing synchronises or
@@ -476,11 +497,12 @@ frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t 
 int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
                 const draw_desc *draws, size_t n,
                 const canvas_desc *canvases_dev, size_t ncanvases, const uint32_t *items_dev,
-                const frame_shape &shape, bool check) {
+                const frame_shape &shape, bool check, fine_config fine = fine_config()) {
     int s;
     if (check) {
         if ((s = L.state.reserve(state_bytes(n))) ||
-            (s = L.counters.reserve(64)) || (s = L.host_counters.reserve(64)))
+            (s = L.counters.reserve(8 * COUNTER_COUNT)) ||
+            (s = L.host_counters.reserve(8 * COUNTER_COUNT)))
             return s;
         uint64_t want_points = std::max<uint64_t>(1 << 22, 64 * shape.in_points);
         uint64_t want_words = std::max<uint64_t>(1 << 22, 8 * shape.in_points + 64 * n);
@@ -513,9 +535,10 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
         F.shadow = (float *)L.shadow.ptr;
         F.shadow_cap = L.shadow.bytes / 4;
         F.counters = (unsigned long long *)L.counters.ptr;
-        F.items = items_dev;
-        F.nitems = shape.items.back();
-        F.segment = fine_segment();
+        F.items = items_dev + (size_t)fine.variant * (ncanvases + 1);
+        F.nitems = shape.items[(size_t)fine.variant * (ncanvases + 1) + ncanvases];
+        F.segment = kFineSegments[fine.variant];
+        F.fine_persistent = fine.persistent ? 1 : 0;
         hipEvent_t *events = nullptr;
         if (d->profiling && !d->capturing) {
             if ((s = profile_events(d, &events)))
@@ -642,6 +665,10 @@ int prepare_program(cih_canvas *cv) {
     program &P = cv->program;
     P.descs.clear();
     P.frames.clear();
+    if (bytes.size() > kPayloadLimit)
+        return fail(CIH_ERROR_BUDGET, "a canvas may hold at most " +
+                                          std::to_string(kPayloadLimit) +
+                                          " bytes of draw packets between flushes");
     size_t at = 0;
     while (at < bytes.size()) {
         draw_desc desc;
@@ -1336,6 +1363,7 @@ struct cih_batch {
     std::vector<void *> graph_buffers; // the lane buffers the graph's kernels use
     bool ran = false;
     bool use_graph = true;
+    fine_config fine; // fine-stage launch shape (cih_batch_configure / cih_batch_tune)
 };
 
 namespace {
@@ -1396,7 +1424,7 @@ int enqueue_batch(cih_batch *b, bool check) {
                              (const draw_desc *)b->draws.ptr + f.first_draw, f.ndraws,
                              (const canvas_desc *)b->frame_canvases.ptr + f.canvases_at,
                              f.ncanvases, (const uint32_t *)b->frame_items.ptr + f.items_at,
-                             f.shape, check)))
+                             f.shape, check, b->fine)))
             return s;
     }
     HIP_TRY(launch_readback_batch(b->all_canvases.ptr, b->prefix_dev.ptr, (int)n, total,
@@ -1494,6 +1522,12 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
         for (int i = 0; i < count; ++i) {
             cih_canvas *v = b->views[(size_t)i];
             size_t base = payload.size();
+            if (base + v->pending.size() > kPayloadLimit) {
+                destroy_batch(b);
+                return fail(CIH_ERROR_BUDGET, "a batch may hold at most " +
+                                                  std::to_string(kPayloadLimit) +
+                                                  " bytes of draw packets");
+            }
             payload.insert(payload.end(), v->pending.begin(), v->pending.end());
             size_t at = base;
             while (at < payload.size()) {
@@ -1667,6 +1701,100 @@ int cih_batch_execute(cih_batch *b, int to_host) {
                                d->stream));
         HIP_TRY(hipStreamSynchronize(d->stream));
     }
+    return CIH_OK;
+}
+
+namespace {
+
+fine_config fine_of(int config) {
+    fine_config f;
+    f.variant = config / 2;
+    f.persistent = (config & 1) != 0;
+    return f;
+}
+
+} // namespace
+
+int cih_batch_configure(cih_batch *b, int config) {
+    if (!b || config < 0 || config >= CIH_FINE_CONFIGS)
+        return fail(CIH_ERROR_INVALID, "invalid fine configuration");
+    cih_device *d = b->device;
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    int s;
+    if ((s = set_device(d)))
+        return s;
+    fine_config f = fine_of(config);
+    if (f.variant == b->fine.variant && f.persistent == b->fine.persistent)
+        return CIH_OK;
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    drop_graph(b); // the graph bakes the fine stage's launch shape
+    b->fine = f;
+    b->ran = false; // the next execute runs checked and captures again
+    return CIH_OK;
+}
+
+int cih_batch_tune(cih_batch *b, int reps, double *ms, int *best) {
+    if (!b || reps < 1)
+        return fail(CIH_ERROR_INVALID, "invalid tuning arguments");
+    cih_device *d = b->device;
+    std::lock_guard<std::recursive_mutex> lock(d->mutex);
+    int s;
+    if ((s = set_device(d)))
+        return s;
+    if (!b->ran && (s = cih_batch_execute(b, 0))) // sizes the arenas
+        return s;
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    hipEvent_t start, stop;
+    HIP_TRY(hipEventCreate(&start));
+    if (hipEventCreate(&stop) != hipSuccess) {
+        hipEventDestroy(start);
+        return fail(CIH_ERROR_DEVICE, "hipEventCreate failed");
+    }
+    double times[CIH_FINE_CONFIGS];
+    int chosen = 0;
+    fine_config keep = b->fine;
+    s = CIH_OK;
+    for (int c = 0; c < CIH_FINE_CONFIGS && s == CIH_OK; ++c) {
+        drop_graph(b);
+        b->fine = fine_of(c);
+        // one untimed execute, then `reps` timed ones (stream launches; the
+        // arenas already fit every configuration)
+        hipError_t e = hipSuccess;
+        if ((s = enqueue_batch(b, false)))
+            break;
+        if ((e = hipEventRecord(start, d->stream)) != hipSuccess) {
+            s = hip_fail(e, "hipEventRecord");
+            break;
+        }
+        for (int r = 0; r < reps && s == CIH_OK; ++r)
+            s = enqueue_batch(b, false);
+        if (s)
+            break;
+        float t = 0.0f;
+        if ((e = hipEventRecord(stop, d->stream)) != hipSuccess ||
+            (e = hipEventSynchronize(stop)) != hipSuccess ||
+            (e = hipEventElapsedTime(&t, start, stop)) != hipSuccess) {
+            s = hip_fail(e, "batch tuning timer");
+            break;
+        }
+        times[c] = (double)t / reps;
+        if (times[c] < times[chosen])
+            chosen = c;
+    }
+    hipEventDestroy(start);
+    hipEventDestroy(stop);
+    drop_graph(b);
+    b->ran = false; // the next execute runs checked and captures again
+    if (s) {
+        b->fine = keep;
+        return s;
+    }
+    b->fine = fine_of(chosen);
+    if (ms)
+        for (int c = 0; c < CIH_FINE_CONFIGS; ++c)
+            ms[c] = times[c];
+    if (best)
+        *best = chosen;
     return CIH_OK;
 }
 
@@ -1870,6 +1998,8 @@ void server_worker(cih_server *sv) {
             view->height = r->height;
             st = cih_canvas_replay(view, r->stream.data(), r->stream.size());
         }
+        if (st == CIH_OK && view->pending.size() > kPayloadLimit)
+            st = CIH_ERROR_BUDGET; // draw records address packets with 32-bit offsets
         if (st == CIH_OK) {
             std::lock_guard<std::recursive_mutex> dlock(d->mutex); // images table
             r->packets.swap(view->pending);
@@ -1926,7 +2056,7 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
         table[i].masks = r->masks ? (uint64_t)(uintptr_t)r->masks->mask_table.ptr : 0;
         table[i].first_draw = 0;
         table[i].ndraws = 0;
-        uint64_t base = payload.size();
+        uint32_t base = (uint32_t)payload.size();
         payload.insert(payload.end(), r->packets.begin(), r->packets.end());
         for (draw_desc dd : r->descs) { // packet offsets move with the concatenation
             dd.points_at += base, dd.subpaths_at += base, dd.dash_at += base;
@@ -1996,7 +2126,8 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
         (s = b->frame_canvases.reserve(sz_fc)) || (s = b->frame_items.reserve(sz_fi)) ||
         (s = b->all_canvases.reserve(sz_tab)) || (s = b->prefix.reserve(sz_pre)) ||
         (s = b->srgb.reserve(4 * total)) || (s = b->srgb_host.reserve(4 * total)) ||
-        (s = b->counters_host.reserve(64 * std::max<size_t>(b->frames.size(), 1))) ||
+        (s = b->counters_host.reserve(8 * COUNTER_COUNT *
+                                      std::max<size_t>(b->frames.size(), 1))) ||
         (s = b->staging.reserve(sz_payload + sz_draws + sz_fc + sz_fi + sz_tab + sz_pre)))
         return s;
     unsigned char *st = (unsigned char *)b->staging.ptr;
@@ -2025,7 +2156,7 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
         ndraws = std::max(ndraws, f.ndraws);
     }
     if ((s = L.state.reserve(state_bytes(ndraws))) ||
-        (s = L.counters.reserve(64)) ||
+        (s = L.counters.reserve(8 * COUNTER_COUNT)) ||
         (s = ensure_arenas(L, std::max<uint64_t>(64 * in_points, 1 << 22),
                            std::max<uint64_t>(8 * in_points + 64 * ndraws, 1 << 22),
                            std::max<uint64_t>(16 * in_points, 1 << 22), shadow)))
@@ -2039,7 +2170,8 @@ int launch_server_batch(cih_server *sv, int k, server_batch *b) {
                              fr.ncanvases, (const uint32_t *)b->frame_items.ptr + fr.items_at,
                              fr.shape, false)))
             return s;
-        HIP_TRY(hipMemcpyAsync((char *)b->counters_host.ptr + 64 * f, L.counters.ptr, 64,
+        HIP_TRY(hipMemcpyAsync((char *)b->counters_host.ptr + 8 * COUNTER_COUNT * f,
+                               L.counters.ptr, 8 * COUNTER_COUNT,
                                hipMemcpyDeviceToHost, L.stream));
     }
     HIP_TRY(launch_readback_batch(b->all_canvases.ptr, b->prefix.ptr, (int)n, total, b->srgb.ptr,
@@ -2058,7 +2190,8 @@ int finish_server_batch(cih_server *sv, int k) {
     bool overflow = false;
     for (size_t f = 0; f < b->frames.size(); ++f) {
         const unsigned long long *c =
-            (const unsigned long long *)((const char *)b->counters_host.ptr + 64 * f);
+            (const unsigned long long *)((const char *)b->counters_host.ptr +
+                                         8 * COUNTER_COUNT * f);
         if (c[COUNTER_OVERFLOW]) {
             overflow = true;
             // grow only the arenas this frame ran out of, to twice its request
@@ -2143,12 +2276,13 @@ void server_scheduler(cih_server *sv) {
                 continue;
             server_batch *b = new server_batch;
             size_t draws = 0;
-            uint64_t pixels = 0;
+            uint64_t pixels = 0, bytes = 0;
             while (!sv->prepared.empty() && (int)b->requests.size() < sv->max_batch) {
                 server_request *r = sv->prepared.front();
                 uint64_t rp = (uint64_t)r->width * (uint64_t)r->height;
                 if (!b->requests.empty() &&
-                    (draws + r->descs.size() > kBatchDrawLimit || pixels + rp > kBatchPixelLimit))
+                    (draws + r->descs.size() > kBatchDrawLimit || pixels + rp > kBatchPixelLimit ||
+                     bytes + r->packets.size() > kPayloadLimit))
                     break;
                 if (!sv->allocator.alloc(r->bytes, r->pool_at))
                     break; // wait for memory: admission by HBM footprint
@@ -2156,6 +2290,7 @@ void server_scheduler(cih_server *sv) {
                 b->requests.push_back(r);
                 draws += r->descs.size();
                 pixels += rp;
+                bytes += r->packets.size();
             }
             if (b->requests.empty()) {
                 delete b;

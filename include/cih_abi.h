@@ -268,6 +268,17 @@ CIH_API int cih_batch_image(cih_batch *batch, int index, unsigned char *image);
 CIH_API int cih_batch_linear(cih_batch *batch, int index, float *rgba);
 /* stats: canvases, draws, frames, pixels. */
 CIH_API int cih_batch_stats(cih_batch *batch, uint64_t *stats);
+/* Fine-stage launch shape of a batch (no reference counterpart): config
+ * 2 * w + p, w = 0, 1, 2 for work items of 256, 128 or 64 pixels of a canvas
+ * row, p = 1 for one persistent grid of waves draining per-XCD item queues
+ * instead of one workgroup per item.  Every configuration gives bit-identical
+ * pixels; config 0 is the default.  cih_batch_tune times all of them on the
+ * batch (`reps` executes each after one untimed), keeps the fastest, writes
+ * each one's milliseconds per execute to ms[CIH_FINE_CONFIGS] (if not NULL)
+ * and its index to *best (if not NULL). */
+#define CIH_FINE_CONFIGS 6
+CIH_API int cih_batch_configure(cih_batch *batch, int config);
+CIH_API int cih_batch_tune(cih_batch *batch, int reps, double *ms, int *best);
 CIH_API int cih_batch_destroy(cih_batch *batch);
 
 /* ---- serving: continuous batching ----------------------------------------

@@ -147,7 +147,8 @@ def emulated_kernels(src):
                     break
             p += 1
         args = src[e + 4:p]
-        out += src[i:k] + "EMU_LAUNCH(%s, (%s), (%s)%s)" % (
+        # the name in parentheses: template arguments may hold commas
+        out += src[i:k] + "EMU_LAUNCH((%s), (%s), (%s)%s)" % (
             name, cfg[0], cfg[1], (", " + args) if args.strip() else "")
         i = p + 1
     out = _shared_regions(out)
@@ -229,6 +230,11 @@ def emulated_runtime(src):
     sub("const uint64_t kBatchPixelLimit = (uint64_t)8 << 20;",
         "const uint64_t kBatchPixelLimit = getenv(\"EMU_BATCH_PIXELS\") ? "
         "strtoull(getenv(\"EMU_BATCH_PIXELS\"), nullptr, 0) : (uint64_t)8 << 20;")
+    # EMU_PAYLOAD_LIMIT lowers the 4 GiB payload limit, so its refusal can be
+    # checked without gigabytes of packets
+    sub("const uint64_t kPayloadLimit = 0xffffffffull;",
+        "const uint64_t kPayloadLimit = getenv(\"EMU_PAYLOAD_LIMIT\") ? "
+        "strtoull(getenv(\"EMU_PAYLOAD_LIMIT\"), nullptr, 0) : 0xffffffffull;")
     sub("const size_t kBatchDrawLimit = 4096;",
         "const size_t kBatchDrawLimit = getenv(\"EMU_BATCH_DRAWS\") ? "
         "strtoull(getenv(\"EMU_BATCH_DRAWS\"), nullptr, 0) : 4096;")

@@ -39,6 +39,7 @@ hipError_t hipEventCreateWithFlags(hipEvent_t *, unsigned);
 hipError_t hipEventDestroy(hipEvent_t);
 hipError_t hipEventRecord(hipEvent_t, hipStream_t);
 hipError_t hipEventQuery(hipEvent_t);
+hipError_t hipEventSynchronize(hipEvent_t);
 hipError_t hipEventElapsedTime(float *, hipEvent_t, hipEvent_t);
 hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode);
 hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *);

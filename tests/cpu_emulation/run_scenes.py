@@ -86,6 +86,84 @@ def batch_limits(device):
             raise AssertionError("a batch above the bound was accepted")
 
 
+def rectangles(n, seed):
+    from canvas_ity_amd.stream import Stream
+    rng = np.random.default_rng(seed)
+    s = Stream(16, 16)
+    for _ in range(n):
+        s.set_color(0, *rng.uniform(0.2, 1.0, 4))
+        x, y = rng.uniform(0, 12, 2)
+        s.fill_rectangle(float(x), float(y), 3.5, 2.25)
+    return s.tobytes()
+
+
+def payload_limit(device):
+    """Run with EMU_PAYLOAD_LIMIT=20000 (the emulation's stand-in for the 4 GiB
+    of 32-bit payload offsets): a canvas holding more packet bytes is refused,
+    a batch whose joined packets exceed it is refused, and the server serves
+    such requests in separate batches with the pixels each gets alone."""
+    assert os.environ.get("EMU_PAYLOAD_LIMIT") == "20000"
+    try:
+        cia.render(rectangles(100, 1), device)
+    except cia.CihError as e:
+        assert "budget" in str(e) and "bytes of draw packets" in str(e), e
+    else:
+        raise AssertionError("a canvas above the payload limit was accepted")
+    pair = [rectangles(40, 2), rectangles(40, 3)]
+    alone = [cia.render(s, device)[0] for s in pair]
+    try:
+        cia.Batch(pair, device)
+    except cia.CihError as e:
+        assert "budget" in str(e) and "bytes of draw packets" in str(e), e
+    else:
+        raise AssertionError("a batch above the payload limit was accepted")
+    server = cia.Server(hbm_budget=1 << 24, max_batch=8, device=device)
+    ids = [server.submit(s) for s in pair]
+    got = {}
+    while len(got) < len(ids):
+        rid = server.wait(timeout_ms=600000)
+        got[rid] = server.result(rid)[0]
+    stats = server.stats()
+    server.stop()
+    for i, rid in enumerate(ids):
+        assert np.array_equal(alone[i], got[rid]), i
+    assert stats["batches"] == 2, stats
+
+
+def fine_configs(device):
+    """Every fine-stage configuration of a batch (item widths 256/128/64
+    pixels, one workgroup per item or a persistent grid draining per-XCD
+    queues) gives bit-identical pixels; the tuner picks one of them and the
+    tuning cache round-trips it."""
+    import tempfile
+    from canvas_ity_amd import tuning
+    all_streams = streams()
+    names = ["plain_small", "large_small", "effects_small", "text_scene_1"]
+    batch_streams = [all_streams[n] for n in names]
+    b = cia.Batch(batch_streams, device)
+    b.execute(to_host=True)
+    ref = [b.linear(i).copy() for i in range(len(names))]
+    for config in range(cia.FINE_CONFIGS):
+        b.configure(config)
+        for _ in range(2):  # the checked run, then the re-run
+            b.execute(to_host=True)
+            for i, n in enumerate(names):
+                got = b.linear(i)
+                assert np.array_equal(ref[i].view(np.uint32), got.view(np.uint32)), (config, n)
+    best, ms = b.tune(reps=1)
+    assert 0 <= best < cia.FINE_CONFIGS and len(ms) == cia.FINE_CONFIGS
+    b.execute(to_host=True)
+    for i in range(len(names)):
+        assert np.array_equal(ref[i].view(np.uint32), b.linear(i).view(np.uint32)), i
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "tune.json")
+        config, key, tuned = tuning.configure(b, batch_streams, path=path, reps=1)
+        assert tuned and tuning.load(path)[key]["config"] == config
+        again, key2, tuned2 = tuning.configure(b, batch_streams, path=path)
+        assert (again, key2, tuned2) == (config, key, False)
+    b.close()
+
+
 def batch_mix(device, n):
     """n canvases of the batch mix (sizes log-uniform 64..1024, 10..400 paths)
     as one batch: each canvas within the parity bar of the oracle."""
@@ -99,7 +177,8 @@ def batch_mix(device, n):
 
 
 MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small,
-         "batch_limits": batch_limits}
+         "batch_limits": batch_limits, "payload_limit": payload_limit,
+         "fine_configs": fine_configs}
 
 
 def mode(name):

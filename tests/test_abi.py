@@ -139,3 +139,34 @@ int main() {
     assert p.returncode == 0, p.stderr
     assert p.stdout.strip() == "zeros"
     assert "no HIP device" in p.stderr
+
+
+def test_tuning_cache_keys_and_round_trip(tmp_path):
+    # the fine-stage autotuner's cache: one entry per batch size class, a
+    # miss tunes the batch, a hit applies the cached configuration
+    from canvas_ity_amd import tuning
+    from canvas_ity_amd.stream import count_draws
+    streams = [cia.scene(4, cia.scene_seed(0xC4, i)) for i in range(6)]
+    key = tuning.size_class([cia.stream_size(s) for s in streams],
+                            [count_draws(s) for s in streams])
+    assert key.startswith("w") and "_n8" in key
+
+    class FakeBatch:
+        def __init__(self):
+            self.applied = None
+            self.tuned = 0
+
+        def tune(self, reps):
+            self.tuned += 1
+            return 3, [1.0, 0.9, 0.8, 0.5, 0.7, 0.6]
+
+        def configure(self, config):
+            self.applied = config
+
+    path = str(tmp_path / "sub" / "tune.json")
+    b = FakeBatch()
+    assert tuning.configure(b, streams, path=path) == (3, key, True)
+    assert tuning.load(path)[key] == {"config": 3, "ms": [1.0, 0.9, 0.8, 0.5, 0.7, 0.6]}
+    b2 = FakeBatch()
+    assert tuning.configure(b2, streams, path=path) == (3, key, False)
+    assert b2.applied == 3 and b2.tuned == 0

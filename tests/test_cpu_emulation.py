@@ -37,7 +37,7 @@ def library():
 STRICT = {"EMU_POISON": "1", "EMU_REDZONES": "1", "EMU_STRICT_STREAMS": "1"}
 
 
-def run(library, names, shuffle=None, strict=False, threads=None):
+def run(library, names, shuffle=None, strict=False, threads=None, extra=None):
     env = dict(os.environ)
     env["LD_PRELOAD"] = emulation.asan_runtime()
     env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=1"
@@ -49,6 +49,8 @@ def run(library, names, shuffle=None, strict=False, threads=None):
         env["EMU_SHUFFLE"] = shuffle
     if strict:
         env.update(STRICT)
+    env.pop("EMU_PAYLOAD_LIMIT", None)
+    env.update(extra or {})
     p = subprocess.run([sys.executable, os.path.join(HERE, "cpu_emulation", "run_scenes.py"),
                         library] + names, env=env, capture_output=True, text=True,
                        timeout=900)
@@ -86,6 +88,13 @@ def test_workgroups_in_random_order(library):
         shuffle="0xC4")
 
 
+def test_payloads_above_the_offset_range_are_refused(library):
+    # draw records address packets with 32-bit offsets; the emulation lowers
+    # the 4 GiB limit to 20000 bytes to check the refusals and the server's
+    # admission without gigabytes of packets
+    run(library, ["payload_limit"], extra={"EMU_PAYLOAD_LIMIT": "20000"})
+
+
 def test_batches_above_the_bounds_are_refused(library):
     # the batch regime that faulted the device (many canvases, tens of
     # thousands of draws in one frame) is refused with CIH_ERROR_BUDGET
@@ -111,3 +120,10 @@ def test_concurrent_workgroups(library):
     # run_scenes.py batch_mix_<n> with the same environment.)
     run(library, ["large_small", "effects_small", "clip_nested", "text_scene_1", "batch_small",
                   "batch_and_server"], strict=True, threads=8)
+
+
+def test_fine_configurations_are_bit_identical(library):
+    # the batch's fine-stage launch shapes (64/128/256-pixel items, one
+    # workgroup per item or a persistent grid draining per-XCD queues) give
+    # the same bits, with workgroups on 8 host threads and the strict checks
+    run(library, ["fine_configs"], strict=True, threads=8)
