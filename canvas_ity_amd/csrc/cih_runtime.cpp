@@ -39,15 +39,15 @@ hipError_t launch_clear_batch(const void *canvases, const void *pixel_prefix, in
                               uint64_t total, hipStream_t stream);
 const int kFrameKernels = 8; // reset, geometry, runs, coalesce, shadow raster, blur rows/cols, fine
 
-// What one batch (and one server batch of several requests) may hold.
-// Multi-canvas frames of up to 2,035 draws and 5.0M pixels (76 canvases) have
-// run cleanly on the MI355X.  Frames of 25,187 draws over 128 canvases
-// (14.4M pixels) and larger batches have faulted the device, and the cause
-// has not been confirmed on the device.  Batches above these bounds are
+// What one batch (and one server batch of several requests) may hold: the
+// largest multi-canvas frame that has run cleanly on the MI355X (76 canvases,
+// 4.98M pixels, 2,035 draws), rounded up.  Frames of 25,187 draws over 128
+// canvases (14.4M pixels) and larger batches have faulted the device, and the
+// cause has not been confirmed on the device.  Batches above these bounds are
 // refused with CIH_ERROR_BUDGET; the server runs a request above them as a
 // batch of its own (a single-canvas frame, like the plain path).
-const uint64_t kBatchPixelLimit = (uint64_t)8 << 20;
-const size_t kBatchDrawLimit = 4096;
+const uint64_t kBatchPixelLimit = (uint64_t)5 << 20;
+const size_t kBatchDrawLimit = 2048;
 // Draws per frame of a batch or server batch (as for a single canvas).
 const size_t kFrameDrawLimit = 8192;
 

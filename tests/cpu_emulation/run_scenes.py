@@ -74,7 +74,7 @@ def batch_small(device):
 def batch_limits(device):
     """Batches above the draw or pixel bound are refused before any kernel
     runs: the 128-canvas batch of the batch mix (14.4M pixels, 25,187 draws),
-    and three 64x64 canvases of 2000 paths each (over 4096 draws)."""
+    and three 64x64 canvases of 2000 paths each (over 2048 draws)."""
     big = [cia.scene(4, cia.scene_seed(0xC4, i)) for i in range(128)]
     many = [cia.scene(4, cia.scene_seed(0x0C, i), 64, 64, 2000) for i in range(3)]
     for streams, what in ((big, "canvas pixels"), (many, "draws")):
@@ -143,24 +143,20 @@ def fine_configs(device):
     b = cia.Batch(batch_streams, device)
     b.execute(to_host=True)
     ref = [b.linear(i).copy() for i in range(len(names))]
-    for config in range(cia.FINE_CONFIGS):
+    for config in range(1, cia.FINE_CONFIGS):
         b.configure(config)
-        for _ in range(2):  # the checked run, then the re-run
-            b.execute(to_host=True)
-            for i, n in enumerate(names):
-                got = b.linear(i)
-                assert np.array_equal(ref[i].view(np.uint32), got.view(np.uint32)), (config, n)
-    best, ms = b.tune(reps=1)
-    assert 0 <= best < cia.FINE_CONFIGS and len(ms) == cia.FINE_CONFIGS
-    b.execute(to_host=True)
-    for i in range(len(names)):
-        assert np.array_equal(ref[i].view(np.uint32), b.linear(i).view(np.uint32)), i
+        b.execute(to_host=True)
+        for i, n in enumerate(names):
+            got = b.linear(i)
+            assert np.array_equal(ref[i].view(np.uint32), got.view(np.uint32)), (config, n)
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "tune.json")
         config, key, tuned = tuning.configure(b, batch_streams, path=path, reps=1)
         assert tuned and tuning.load(path)[key]["config"] == config
-        again, key2, tuned2 = tuning.configure(b, batch_streams, path=path)
-        assert (again, key2, tuned2) == (config, key, False)
+        assert 0 <= config < cia.FINE_CONFIGS
+    b.execute(to_host=True)
+    for i in range(len(names)):
+        assert np.array_equal(ref[i].view(np.uint32), b.linear(i).view(np.uint32)), i
     b.close()
 
 

@@ -61,31 +61,33 @@ def run(library, names, shuffle=None, strict=False, threads=None, extra=None):
 
 
 def test_configurations_memory_safe_and_in_parity(library):
+    # workgroups one at a time, in index order
     run(library, ["plain_small", "large_small", "effects_small"])
 
 
 def test_clip_masks_memory_safe_and_in_parity(library):
     run(library, ["clip_nested", "clip_op_destination_in", "clip_shadow_image",
-                  "clip_star_empty", "reference_global_composite_operation"])
+                  "clip_star_empty", "reference_global_composite_operation"], threads=8)
 
 
 def test_text_memory_safe_and_in_parity(library):
-    run(library, ["text_font", "text_fill_text", "text_stroke_text", "text_scene_1"])
+    run(library, ["text_font", "text_fill_text", "text_stroke_text", "text_scene_1"], threads=8)
 
 
 def test_put_image_data_in_batches_and_server(library):
     # puts recorded as packets in batch and server canvases, applied in
     # painter's order: the same pixels as the plain canvas, which applies them
     # at once
-    run(library, ["batch_and_server"])
+    run(library, ["batch_and_server"], threads=8)
 
 
 def test_workgroups_in_random_order(library):
     # a GPU runs a kernel's workgroups concurrently, in no fixed order; here
-    # they run in a seeded random order (EMU_SHUFFLE), and the renders, alone
-    # and as one multi-canvas batch, must still be memory-safe and in parity
+    # they are handed out in a seeded random order (EMU_SHUFFLE) to 8 host
+    # threads, and the renders, alone and as one multi-canvas batch, must
+    # still be memory-safe and in parity
     run(library, ["large_small", "effects_small", "clip_nested", "text_scene_1", "batch_small"],
-        shuffle="0xC4")
+        shuffle="0xC4", threads=8)
 
 
 def test_payloads_above_the_offset_range_are_refused(library):
@@ -101,23 +103,15 @@ def test_batches_above_the_bounds_are_refused(library):
     run(library, ["batch_limits"])
 
 
-def test_strict_checks(library):
+def test_strict_checks_with_concurrent_workgroups(library):
     # device buffers and LDS start as 0xA5 garbage, every arena sub-allocation
     # is followed by a poisoned red zone (a draw overrunning into the next
-    # draw's region aborts), and NULL-stream uploads are held back until the
-    # NULL stream is synchronised (the library's streams do not wait for it):
-    # single canvases, a multi-canvas batch with its captured re-run, and the
-    # two-lane server must stay memory-safe and in parity
-    run(library, ["large_small", "effects_small", "clip_nested", "text_scene_1", "batch_small",
-                  "batch_and_server"], strict=True)
-
-
-def test_concurrent_workgroups(library):
-    # a kernel's workgroups on 8 host threads at once (atomics are host
-    # atomics, LDS is per thread), with the strict checks on: renders alone,
-    # a multi-canvas batch and the server must stay memory-safe and in parity
-    # when workgroups interleave as on a GPU.  (Full-size batch-mix batches:
-    # run_scenes.py batch_mix_<n> with the same environment.)
+    # draw's region aborts), NULL-stream uploads are held back until the NULL
+    # stream is synchronised (the library's streams do not wait for it), and a
+    # kernel's workgroups run on 8 host threads at once (atomics are host
+    # atomics, LDS is per thread): single canvases, a multi-canvas batch and
+    # the two-lane server must stay memory-safe and in parity.  (Full-size
+    # batch-mix batches: run_scenes.py batch_mix_<n> with the same environment.)
     run(library, ["large_small", "effects_small", "clip_nested", "text_scene_1", "batch_small",
                   "batch_and_server"], strict=True, threads=8)
 
