@@ -160,6 +160,26 @@ def fine_configs(device):
     b.close()
 
 
+def canvas_execute(device):
+    """The benchmark's path: a flushed canvas program run again from HBM
+    (cih_canvas_execute, unchecked) gives the first render's pixels, and
+    its statistics pass counts composited pixels."""
+    stream = streams()["large_small"]
+    w, h = cia.stream_size(stream)
+    c = cia.Canvas(w, h, device)
+    c.replay(stream)
+    first8, firstl = c.rgba8().copy(), c.linear().copy()
+    for _ in range(2):
+        c.execute(clear=True, readback=True)
+        device.synchronize()
+        assert np.array_equal(firstl.view(np.uint32), c.linear().view(np.uint32))
+        assert np.array_equal(first8, c.rgba8())
+    stats = c.stats()
+    assert stats["pixel_draws"] > 0 and stats["runs"] > 0 and stats["points"] > 0, stats
+    assert np.array_equal(firstl.view(np.uint32), c.linear().view(np.uint32))
+    c.close()
+
+
 def batch_mix(device, n):
     """n canvases of the batch mix (sizes log-uniform 64..1024, 10..400 paths)
     as one batch: each canvas within the parity bar of the oracle."""
@@ -174,7 +194,7 @@ def batch_mix(device, n):
 
 MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small,
          "batch_limits": batch_limits, "payload_limit": payload_limit,
-         "fine_configs": fine_configs}
+         "fine_configs": fine_configs, "canvas_execute": canvas_execute}
 
 
 def mode(name):

@@ -61,8 +61,9 @@ def run(library, names, shuffle=None, strict=False, threads=None, extra=None):
 
 
 def test_configurations_memory_safe_and_in_parity(library):
-    # workgroups one at a time, in index order
-    run(library, ["plain_small", "large_small", "effects_small"])
+    # workgroups one at a time, in index order; and the benchmark's path, a
+    # flushed canvas run again from HBM
+    run(library, ["plain_small", "large_small", "effects_small", "canvas_execute"])
 
 
 def test_clip_masks_memory_safe_and_in_parity(library):

@@ -292,6 +292,21 @@ def test_serve_requests(device, index):
     check(cia.scene(5, cia.scene_seed(0xC5, index), 0, 0, 300, flags), "request %d" % index)
 
 
+def test_execute_repeats_the_render(device):
+    # the benchmark's path: the flushed program run again from HBM gives the
+    # same bits, and the statistics pass counts the composited pixels
+    stream = cia.scene(2, 0x33C2, 300, 200, 120)
+    c = cia.Canvas(300, 200, device)
+    c.replay(stream)
+    first = c.linear().copy()
+    for _ in range(3):
+        c.execute(clear=True, readback=True)
+        device.synchronize()
+        assert np.array_equal(first.view(np.uint32), c.linear().view(np.uint32))
+    assert c.stats()["pixel_draws"] > 0
+    c.close()
+
+
 def test_deterministic(device):
     s = cia.scene(2, 0x5C2, 512, 512, 300)
     a8, al = gpu(s)
