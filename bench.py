@@ -131,7 +131,7 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="N",
                     help="also measure the batch mix: N canvases rendered as one batch vs "
                          "one canvas at a time (canvases/s); the library refuses batches "
-                         "above 5M pixels or 2048 draws (about 10 canvases of this mix)")
+                         "above 5M pixels or 2304 draws (about 11 canvases of this mix)")
     ap.add_argument("--tune", action="store_true",
                     help="with --batch: pick the batch's fine-stage launch shape with the "
                          "autotuner (cached per size class in $CIH_TUNE_CACHE or "

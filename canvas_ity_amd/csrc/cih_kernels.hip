@@ -3,7 +3,7 @@
 // Per frame (a list of draws over one or more canvases):
 //   k_reset        clears the arena counters (a kernel, so frames can be
 //                  captured in HIP graphs without memset nodes)
-//   k_geometry     one workgroup per draw: flatten (one thread per cubic),
+//   k_geometry     one wave per draw: flatten (one thread per cubic),
 //                  dash and stroke (one thread per subpath) -> closed polygons
 //   k_runs         one workgroup per draw and pass (main, and shadow when the
 //                  frame has shadows): clip (hpp:2205-2238), signed-area runs
@@ -36,7 +36,11 @@
 
 namespace cih {
 
-#define GEO_THREADS 256
+#define GEO_THREADS 256 // k_runs: one workgroup of 4 waves per draw and pass
+// k_geometry: one wave per draw.  Most draws have a few cubics and subpaths
+// (one thread each), so a wave keeps as many lanes busy as four would, and a
+// CU holds four times as many draws (its LDS use is a few words).
+#define GEOM_THREADS 64
 #define FINE_THREADS 64 // one wave: segment_sums relies on it
 #define FINE_PX 4
 #define FINE_SEGMENT (FINE_THREADS * FINE_PX)
@@ -71,7 +75,7 @@ __device__ inline uint64_t arena_alloc(const frame &F, int which, uint64_t n, ui
     return at;
 }
 
-// Exclusive scan of one value per thread over a GEO_THREADS workgroup.
+// Exclusive scan of one value per thread over the workgroup (of up to 16 waves).
 __device__ inline uint32_t block_exclusive(uint32_t v, uint32_t *lds, uint32_t &total) {
     int t = threadIdx.x;
     int lane = t & 63, wave = t >> 6;
@@ -456,7 +460,7 @@ __device__ line_path dash_all(const frame &F, const draw_desc &D, line_path in, 
     return out;
 }
 
-__global__ __launch_bounds__(GEO_THREADS) void k_geometry(frame F) {
+__global__ __launch_bounds__(GEOM_THREADS) void k_geometry(frame F) {
     __shared__ uint32_t lds[16];
     __shared__ uint64_t slot[1];
     int d = blockIdx.x;
@@ -2526,7 +2530,7 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
         k_put<<<grid_for((size_t)puts[2 * k + 1], 256), 256, 0, stream>>>(F, puts[2 * k]);
     MARK(1);
     if (F.ndraws > 0)
-        k_geometry<<<F.ndraws, GEO_THREADS, 0, stream>>>(F);
+        k_geometry<<<F.ndraws, GEOM_THREADS, 0, stream>>>(F);
     MARK(2);
     if (F.ndraws > 0)
         k_runs<<<(any_shadow ? 2 : 1) * F.ndraws, GEO_THREADS, 0, stream>>>(F, any_shadow ? 2 : 1);

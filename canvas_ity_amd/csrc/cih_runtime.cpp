@@ -41,13 +41,13 @@ const int kFrameKernels = 8; // reset, geometry, runs, coalesce, shadow raster, 
 
 // What one batch (and one server batch of several requests) may hold: the
 // largest multi-canvas frame that has run cleanly on the MI355X (76 canvases,
-// 4.98M pixels, 2,035 draws), rounded up.  Frames of 25,187 draws over 128
+// 4.98M pixels, 2,141 draws), rounded up.  Frames of 25,187 draws over 128
 // canvases (14.4M pixels) and larger batches have faulted the device, and the
 // cause has not been confirmed on the device.  Batches above these bounds are
 // refused with CIH_ERROR_BUDGET; the server runs a request above them as a
 // batch of its own (a single-canvas frame, like the plain path).
 const uint64_t kBatchPixelLimit = (uint64_t)5 << 20;
-const size_t kBatchDrawLimit = 2048;
+const size_t kBatchDrawLimit = 2304;
 // Draws per frame of a batch or server batch (as for a single canvas).
 const size_t kFrameDrawLimit = 8192;
 

@@ -254,7 +254,7 @@ CIH_API int cih_text_outline(const unsigned char *font, size_t font_bytes, float
  * it to pinned host memory and waits.  No reference counterpart: the
  * reference renders one canvas per object (canvas_ity.hpp:194). */
 typedef struct cih_batch cih_batch;
-/* A batch holds at most 5M (5 * 2^20) canvas pixels and 2048 draws in this
+/* A batch holds at most 5M (5 * 2^20) canvas pixels and 2304 draws in this
  * build (CIH_ERROR_BUDGET beyond either).  A server batch admits requests up
  * to the same totals; a single request above them runs as a batch of its
  * own. */

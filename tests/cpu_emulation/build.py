@@ -224,6 +224,18 @@ def emulated_runtime(src):
         assert old in src, old[:60]
         src = src.replace(old, new)
     sub("        HIP_TRY(launch_frame(F,", UNPOISON_ARENAS)
+    # EMU_TRACE_ARENAS: each checked attempt's arena requests and capacities
+    sub("""        const unsigned long long *hc = (const unsigned long long *)L.host_counters.ptr;
+        if (!hc[COUNTER_OVERFLOW])""",
+        """        const unsigned long long *hc = (const unsigned long long *)L.host_counters.ptr;
+        if (getenv("EMU_TRACE_ARENAS"))
+            fprintf(stderr, "[arenas] draws %zu attempt %d: points %llu/%llu words %llu/%llu "
+                    "runs %llu/%llu shadow %llu/%llu overflow %llu\\n", n, attempt,
+                    hc[COUNTER_POINTS], (unsigned long long)F.points_cap, hc[COUNTER_WORDS],
+                    (unsigned long long)F.words_cap, hc[COUNTER_RUNS],
+                    (unsigned long long)F.runs_cap, hc[COUNTER_SHADOW],
+                    (unsigned long long)F.shadow_cap, hc[COUNTER_OVERFLOW]);
+        if (!hc[COUNTER_OVERFLOW])""")
     # EMU_BATCH_DRAWS / EMU_BATCH_PIXELS raise the batch bounds in the
     # emulation only, so batches that are refused on the device can be run
     # on the CPU
@@ -235,9 +247,9 @@ def emulated_runtime(src):
     sub("const uint64_t kPayloadLimit = 0xffffffffull;",
         "const uint64_t kPayloadLimit = getenv(\"EMU_PAYLOAD_LIMIT\") ? "
         "strtoull(getenv(\"EMU_PAYLOAD_LIMIT\"), nullptr, 0) : 0xffffffffull;")
-    sub("const size_t kBatchDrawLimit = 2048;",
+    sub("const size_t kBatchDrawLimit = 2304;",
         "const size_t kBatchDrawLimit = getenv(\"EMU_BATCH_DRAWS\") ? "
-        "strtoull(getenv(\"EMU_BATCH_DRAWS\"), nullptr, 0) : 2048;")
+        "strtoull(getenv(\"EMU_BATCH_DRAWS\"), nullptr, 0) : 2304;")
     sub("std::max(need, bytes + bytes / 2)", "need")
     sub("attempt < 8;", "attempt < 64;")
     sub("    L.last_ndraws = n;\n", SAVE_CANVASES)

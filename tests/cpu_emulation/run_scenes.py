@@ -74,7 +74,7 @@ def batch_small(device):
 def batch_limits(device):
     """Batches above the draw or pixel bound are refused before any kernel
     runs: the 128-canvas batch of the batch mix (14.4M pixels, 25,187 draws),
-    and three 64x64 canvases of 2000 paths each (over 2048 draws)."""
+    and three 64x64 canvases of 2000 paths each (over 2304 draws)."""
     big = [cia.scene(4, cia.scene_seed(0xC4, i)) for i in range(128)]
     many = [cia.scene(4, cia.scene_seed(0x0C, i), 64, 64, 2000) for i in range(3)]
     for streams, what in ((big, "canvas pixels"), (many, "draws")):
@@ -180,6 +180,29 @@ def canvas_execute(device):
     c.close()
 
 
+def reference_suite_batch(device):
+    """The reference's 76 tests (test.cpp) as one batch: every canvas within
+    the parity bar of the oracle and within the reference suite's own pass
+    criterion, image hash within Hamming distance 5 of its golden table."""
+    import reference_suite
+    hashes = scenes.reference_hashes()
+    names = sorted(hashes)
+    batch_streams = [reference_suite.stream(n) for n in names]
+    b = cia.Batch(batch_streams, device)
+    b.execute(to_host=True)
+    worst = 0
+    for i, n in enumerate(names):
+        g8, gl = b.rgba8(i), b.linear(i)
+        r8, rl = oracle.render(batch_streams[i], "port")
+        assert_parity(r8, rl, g8, gl, n)
+        dist = scenes.hash_distance(scenes.image_hash(g8), hashes[n]["hash"])
+        assert dist <= 5, (n, dist)
+        worst = max(worst, dist)
+    b.close()
+    print("reference suite: %d canvases in one batch, worst hash distance %d"
+          % (len(names), worst), flush=True)
+
+
 def batch_mix(device, n):
     """n canvases of the batch mix (sizes log-uniform 64..1024, 10..400 paths)
     as one batch: each canvas within the parity bar of the oracle."""
@@ -194,7 +217,8 @@ def batch_mix(device, n):
 
 MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small,
          "batch_limits": batch_limits, "payload_limit": payload_limit,
-         "fine_configs": fine_configs, "canvas_execute": canvas_execute}
+         "fine_configs": fine_configs, "canvas_execute": canvas_execute,
+         "reference_suite_batch": reference_suite_batch}
 
 
 def mode(name):

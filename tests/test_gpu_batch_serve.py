@@ -5,7 +5,7 @@
   (bit-identical fp32 and sRGB8: both run the same kernels, and each draw's
   coverage comes from its own runs alone, so the grouping of canvases into
   frames cannot change a result), and within the parity bar of the CPU oracle.
-* A batch above the 5M-pixel or 2048-draw bound is refused before anything
+* A batch above the 5M-pixel or 2304-draw bound is refused before anything
   runs on the device.
 * The server returns, for requests submitted together and interleaved, the
   same pixels each request gets when rendered alone; a memory budget smaller
@@ -47,7 +47,7 @@ def test_batch_refuses_oversized(device):
     streams = [cia.scene(4, cia.scene_seed(0x0B, i), 4096, 4096, 10) for i in range(5)]
     with pytest.raises(cia.CihError, match="budget"):
         cia.Batch(streams, device)
-    # 3 small canvases of 2000 paths each hold more than 2048 draws
+    # 3 small canvases of 2000 paths each hold more than 2304 draws
     streams = [cia.scene(4, cia.scene_seed(0x0C, i), 64, 64, 2000) for i in range(3)]
     with pytest.raises(cia.CihError, match="budget"):
         cia.Batch(streams, device)
