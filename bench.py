@@ -12,8 +12,10 @@ per second.
 
 The JSON line also carries the roofline of the dominant kernel (its time
 from HIP events recorded by the library on its own stream during the timed
-steps) and a CPU baseline: the reference canvas_ity compiled from source
-(oracle/_ref) timed on this host on a bounded sample of the same workload.
+steps), an end-to-end figure with the PCIe transfers, a CPU baseline (the
+reference canvas_ity compiled from source, oracle/_ref, timed on this host)
+and the batch mix: 10 canvases rendered as one batch against the same
+canvases one at a time.
 """
 import argparse
 import json
@@ -128,10 +130,11 @@ def main():
     ap.add_argument("--end-to-end", type=int, default=3, metavar="K",
                     help="canvases timed from command stream to sRGB8 in host memory "
                          "(replay, upload, render, copy back); 0: skip")
-    ap.add_argument("--batch", type=int, default=0, metavar="N",
+    ap.add_argument("--batch", type=int, default=10, metavar="N",
                     help="also measure the batch mix: N canvases rendered as one batch vs "
-                         "one canvas at a time (canvases/s); the library refuses batches "
-                         "above 5M pixels or 2304 draws (about 11 canvases of this mix)")
+                         "one canvas at a time (canvases/s; 0: skip); the library refuses "
+                         "batches above 5M pixels or 2304 draws (about 11 canvases of this "
+                         "mix)")
     ap.add_argument("--tune", action="store_true",
                     help="with --batch: pick the batch's fine-stage launch shape with the "
                          "autotuner (cached per size class in $CIH_TUNE_CACHE or "

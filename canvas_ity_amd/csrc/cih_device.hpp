@@ -99,8 +99,13 @@ enum counter {
     COUNTER_OVERFLOW = 4,
     COUNTER_QUEUE = 5, // coalesce work list: (entries << 40) + rows
     COUNTER_PIXEL_DRAWS = 6, // pixels composited by k_fine (main and shadow layers)
-    COUNTER_FINE_QUEUE = 8,  // 8 cursors of the persistent fine kernel's item ranges
-    COUNTER_COUNT = 16
+    // draws whose geometry (k_geometry) and passes whose scan conversion
+    // (k_runs) completed: after an overflow they tell the host which share
+    // of the frame its arena requests cover
+    COUNTER_GEOMETRY_DONE = 7,
+    COUNTER_FINE_QUEUE = 8, // 8 cursors of the persistent fine kernel's item ranges
+    COUNTER_RUNS_DONE = 16,
+    COUNTER_COUNT = 17
 };
 
 struct frame {

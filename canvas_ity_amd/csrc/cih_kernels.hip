@@ -508,6 +508,7 @@ __global__ __launch_bounds__(GEOM_THREADS) void k_geometry(frame F) {
         S.poly_counts = (uint64_t)(uintptr_t)lp.sub;
         S.poly_npoints = lp.npts;
         S.poly_nsub = lp.nsub;
+        atomicAdd(&F.counters[COUNTER_GEOMETRY_DONE], 1ull);
     }
 }
 
@@ -812,6 +813,7 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F, int passes) {
         S.runs = rb;
         S.minx = s_b[0];
         S.maxx = s_b[1];
+        atomicAdd(&F.counters[COUNTER_RUNS_DONE], 1ull);
         return;
     }
     S.srow0 = row0;
@@ -853,6 +855,8 @@ __global__ __launch_bounds__(GEO_THREADS) void k_runs(frame F, int passes) {
     if (img == ~0ull) {
         S.width = S.height = 0;
         img = 0;
+    } else {
+        atomicAdd(&F.counters[COUNTER_RUNS_DONE], 1ull);
     }
     S.image = img;
     S.image2 = img + size;

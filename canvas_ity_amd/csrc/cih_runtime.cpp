@@ -244,6 +244,7 @@ struct frame_shape {
     std::vector<int32_t> clips;  // (draw, canvas height) of each clip draw, in order
     std::vector<int32_t> puts;   // (draw, texel count) of each put, in order
     bool masked = false;         // some draw is composited under a clip mask
+    size_t nshadowed = 0;        // draws with a shadow pass
 };
 
 // A canvas' decoded draw list, resident in HBM so it can be run again, with
@@ -477,6 +478,7 @@ frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t 
             f.masked = f.masked || (dd.clip_mask != 0 && dd.kind != CIH_DRAW_CLIP);
             if (dd.shadow) {
                 f.any_shadow = true;
+                f.nshadowed++;
                 int b = dd.border;
                 f.max_rows = std::max(f.max_rows, cd.height + 2 * b + 1);
                 f.max_cols = std::max(f.max_cols, cd.width + 2 * b + 1);
@@ -489,6 +491,31 @@ frame_shape shape_of(const draw_desc *host, const canvas_desc *canvases, size_t 
 }
 
 // Launches the kernels of one frame whose canvas list and work items are
+// Arena capacities for the next attempt of a frame that overflowed.  Once an
+// arena is full, draws that start later return without asking, so the
+// counters hold only the requests of the draws that got through (and those in
+// flight).  Each overflowed arena is grown to its request scaled by the
+// share of the frame still missing: geometry's when k_geometry did not
+// complete every draw, else scan conversion's (every main pass, and the
+// shadow pass of each shadowed draw when the frame has shadows) with a 1.5x
+// margin; at least twice the request, at most 16 times.  caps[] holds points,
+// words, runs and shadow capacities and is updated in place.
+void grow_after_overflow(const unsigned long long *hc, size_t ndraws, size_t passes,
+                         uint64_t caps[4]) {
+    static const int which[4] = {COUNTER_POINTS, COUNTER_WORDS, COUNTER_RUNS, COUNTER_SHADOW};
+    double geometry = hc[COUNTER_GEOMETRY_DONE]
+                          ? (double)ndraws / (double)hc[COUNTER_GEOMETRY_DONE]
+                          : 16.0;
+    double runs = hc[COUNTER_RUNS_DONE] ? (double)passes / (double)hc[COUNTER_RUNS_DONE] : 16.0;
+    double scale = hc[COUNTER_GEOMETRY_DONE] < ndraws ? geometry : runs;
+    scale = std::min(16.0, std::max(2.0, 1.5 * scale)); // draws differ: a margin
+    for (int k = 0; k < 4; ++k) {
+        uint64_t asked = hc[which[k]];
+        if (asked >= caps[k])
+            caps[k] = std::max<uint64_t>(caps[k], (uint64_t)((double)asked * scale));
+    }
+}
+
 // Launches the kernels of one frame whose canvas list and work items are
 // already in HBM.  With `check`, waits for it and, if an arena ran out, grows
 // the arenas to what the frame asked for and runs it again (kernels write no
@@ -558,14 +585,9 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
         if (!hc[COUNTER_OVERFLOW])
             return CIH_OK;
         d->retries++;
-        uint64_t np = std::max<uint64_t>(hc[COUNTER_POINTS], F.points_cap);
-        uint64_t nw = std::max<uint64_t>(hc[COUNTER_WORDS], F.words_cap);
-        uint64_t nr = std::max<uint64_t>(hc[COUNTER_RUNS], F.runs_cap);
-        uint64_t ns = std::max<uint64_t>(hc[COUNTER_SHADOW], F.shadow_cap);
-        if ((s = ensure_arenas(L, hc[COUNTER_POINTS] >= F.points_cap ? 2 * np : np,
-                               hc[COUNTER_WORDS] >= F.words_cap ? 2 * nw : nw,
-                               hc[COUNTER_RUNS] >= F.runs_cap ? 2 * nr : nr,
-                               hc[COUNTER_SHADOW] >= F.shadow_cap ? 2 * ns : ns)))
+        uint64_t caps[4] = {F.points_cap, F.words_cap, F.runs_cap, F.shadow_cap};
+        grow_after_overflow(hc, n, n + (shape.any_shadow ? shape.nshadowed : 0), caps);
+        if ((s = ensure_arenas(L, caps[0], caps[1], caps[2], caps[3])))
             return s;
     }
     return fail(CIH_ERROR_OUT_OF_MEMORY, "frame did not fit after repeated arena growth");
@@ -2194,12 +2216,13 @@ int finish_server_batch(cih_server *sv, int k) {
                                          8 * COUNTER_COUNT * f);
         if (c[COUNTER_OVERFLOW]) {
             overflow = true;
-            // grow only the arenas this frame ran out of, to twice its request
-            auto grow = [](uint64_t asked, uint64_t cap) { return asked > cap ? 2 * asked : cap; };
-            int s = ensure_arenas(L, grow(c[COUNTER_POINTS], L.points.bytes / 8),
-                                  grow(c[COUNTER_WORDS], L.words.bytes / 4),
-                                  grow(c[COUNTER_RUNS], run_capacity(L)),
-                                  grow(c[COUNTER_SHADOW], L.shadow.bytes / 4));
+            // grow only the arenas this frame ran out of (grow_after_overflow)
+            const server_batch::frame_rec &fr = b->frames[f];
+            uint64_t caps[4] = {L.points.bytes / 8, L.words.bytes / 4, run_capacity(L),
+                                L.shadow.bytes / 4};
+            grow_after_overflow(c, fr.ndraws,
+                                fr.ndraws + (fr.shape.any_shadow ? fr.shape.nshadowed : 0), caps);
+            int s = ensure_arenas(L, caps[0], caps[1], caps[2], caps[3]);
             if (s)
                 return s;
         }

@@ -224,6 +224,23 @@ def emulated_runtime(src):
         assert old in src, old[:60]
         src = src.replace(old, new)
     sub("        HIP_TRY(launch_frame(F,", UNPOISON_ARENAS)
+    # EMU_SMALL_ARENAS: first arenas of 1024 entries, so that frames overflow
+    # and the growth after an overflow (checked frames and server batches) runs
+    small = "(getenv(\"EMU_SMALL_ARENAS\") ? (uint64_t)1 << 10 : (uint64_t)1 << 22)"
+    sub("        uint64_t want_points = std::max<uint64_t>(1 << 22, 64 * shape.in_points);\n"
+        "        uint64_t want_words = std::max<uint64_t>(1 << 22, 8 * shape.in_points + 64 * n);\n"
+        "        uint64_t want_runs = std::max<uint64_t>(1 << 22, 16 * shape.in_points);",
+        "        uint64_t want_points = getenv(\"EMU_SMALL_ARENAS\") ? 1024 : std::max<uint64_t>(%s, 64 * shape.in_points);\n"
+        "        uint64_t want_words = getenv(\"EMU_SMALL_ARENAS\") ? 1024 : std::max<uint64_t>(%s, 8 * shape.in_points + 64 * n);\n"
+        "        uint64_t want_runs = getenv(\"EMU_SMALL_ARENAS\") ? 1024 : std::max<uint64_t>(%s, 16 * shape.in_points);"
+        % (small, small, small))
+    sub("        (s = ensure_arenas(L, std::max<uint64_t>(64 * in_points, 1 << 22),\n"
+        "                           std::max<uint64_t>(8 * in_points + 64 * ndraws, 1 << 22),\n"
+        "                           std::max<uint64_t>(16 * in_points, 1 << 22), shadow)))",
+        "        (s = ensure_arenas(L, getenv(\"EMU_SMALL_ARENAS\") ? 1024 : std::max<uint64_t>(64 * in_points, 1 << 22),\n"
+        "                           getenv(\"EMU_SMALL_ARENAS\") ? 1024 : std::max<uint64_t>(8 * in_points + 64 * ndraws, 1 << 22),\n"
+        "                           getenv(\"EMU_SMALL_ARENAS\") ? 1024 : std::max<uint64_t>(16 * in_points, 1 << 22),\n"
+        "                           getenv(\"EMU_SMALL_ARENAS\") ? 1024 : shadow)))")
     # EMU_TRACE_ARENAS: each checked attempt's arena requests and capacities
     sub("""        const unsigned long long *hc = (const unsigned long long *)L.host_counters.ptr;
         if (!hc[COUNTER_OVERFLOW])""",

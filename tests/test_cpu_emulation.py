@@ -49,7 +49,8 @@ def run(library, names, shuffle=None, strict=False, threads=None, extra=None):
         env["EMU_SHUFFLE"] = shuffle
     if strict:
         env.update(STRICT)
-    env.pop("EMU_PAYLOAD_LIMIT", None)
+    for k in ("EMU_PAYLOAD_LIMIT", "EMU_SMALL_ARENAS", "EMU_TRACE_ARENAS"):
+        env.pop(k, None)
     env.update(extra or {})
     p = subprocess.run([sys.executable, os.path.join(HERE, "cpu_emulation", "run_scenes.py"),
                         library] + names, env=env, capture_output=True, text=True,
@@ -58,6 +59,7 @@ def run(library, names, shuffle=None, strict=False, threads=None, extra=None):
     assert p.returncode == 0, report
     for n in names:
         assert "%s ok" % n in p.stdout, report
+    return p.stdout + p.stderr
 
 
 def test_configurations_memory_safe_and_in_parity(library):
@@ -122,3 +124,15 @@ def test_fine_configurations_are_bit_identical(library):
     # workgroup per item or a persistent grid draining per-XCD queues) give
     # the same bits, with workgroups on 8 host threads and the strict checks
     run(library, ["fine_configs"], strict=True, threads=8)
+
+
+def test_arena_growth_after_overflow(library):
+    # first arenas of 1024 entries: every frame overflows, and the host grows
+    # the arenas it ran out of (scaled by the share of draws that got through)
+    # and runs the frame again, for checked frames and for server batches;
+    # the renders must stay memory-safe and in parity
+    out = run(library, ["large_small", "effects_small", "clip_nested", "batch_small",
+                        "batch_and_server"], threads=8,
+              extra={"EMU_SMALL_ARENAS": "1", "EMU_TRACE_ARENAS": "1"})
+    attempts = [line for line in out.splitlines() if line.startswith("[arenas]")]
+    assert sum(" overflow 1" in line for line in attempts) >= 3, out[-3000:]
