@@ -132,9 +132,9 @@ def main():
                          "(replay, upload, render, copy back); 0: skip")
     ap.add_argument("--batch", type=int, default=10, metavar="N",
                     help="also measure the batch mix: N canvases rendered as one batch vs "
-                         "one canvas at a time (canvases/s; 0: skip); the library refuses "
-                         "batches above 5M pixels or 2304 draws (about 11 canvases of this "
-                         "mix)")
+                         "one canvas at a time (canvases/s; 0: skip); the batch cuts its "
+                         "draws into frames of at most 2048 draws and 2M pixels (about 10 "
+                         "canvases of this mix each) and holds at most 2^30 pixels")
     ap.add_argument("--tune", action="store_true",
                     help="with --batch: pick the batch's fine-stage launch shape with the "
                          "autotuner (cached per size class in $CIH_TUNE_CACHE or "
@@ -162,9 +162,11 @@ def main():
         device = cia.default_device()
         line = {}
         if args.batch:
-            line["batch"] = bench_batch(cia, device, args.batch, tune=args.tune)
+            optional_leg(line, "batch",
+                         lambda: bench_batch(cia, device, args.batch, tune=args.tune))
         if args.serve:
-            line["serving"] = bench_serve(cia, device, args.serve, args.requests)
+            optional_leg(line, "serving",
+                         lambda: bench_serve(cia, device, args.serve, args.requests))
         print(json.dumps(line))
         return 0
     stream = cia.scene(2, SEED)
@@ -220,18 +222,30 @@ def main():
                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                         "traffic_source": traffic_source, "pixel_draws": pixel_draws}
     if args.end_to_end > 0:
-        line["end_to_end"] = bench_end_to_end(cia, device, stream, args.end_to_end)
+        optional_leg(line, "end_to_end",
+                     lambda: bench_end_to_end(cia, device, stream, args.end_to_end))
     if args.cpu_sample > 0:
         threads = args.cpu_threads or cpu_threads()
-        line["cpu_baseline"] = cpu_baseline(cia, stream, width, height, args.cpu_sample,
-                                            threads)
+        optional_leg(line, "cpu_baseline",
+                     lambda: cpu_baseline(cia, stream, width, height, args.cpu_sample, threads))
     canvas.close()
     if args.batch:
-        line["batch"] = bench_batch(cia, device, args.batch)
+        optional_leg(line, "batch", lambda: bench_batch(cia, device, args.batch, tune=args.tune))
     if args.serve:
-        line["serving"] = bench_serve(cia, device, args.serve, args.requests)
+        optional_leg(line, "serving",
+                     lambda: bench_serve(cia, device, args.serve, args.requests))
     print(json.dumps(line))
     return 0
+
+
+def optional_leg(line, key, measure):
+    """Runs one of the line's further measurements; an error is recorded in
+    its place (and on stderr) so the headline line is still printed."""
+    try:
+        line[key] = measure()
+    except Exception as e:  # noqa: BLE001 -- the line is printed either way
+        print("bench.py: %s failed: %s: %s" % (key, type(e).__name__, e), file=sys.stderr)
+        line[key] = {"error": "%s: %s" % (type(e).__name__, e)}
 
 
 def bench_end_to_end(cia, device, stream, k):
@@ -267,6 +281,7 @@ def bench_batch(cia, device, n, seed=0xC4, reps=3, tune=False):
     except cia.CihError as e:
         return {"workload": "batch mix: %d canvases (seed %#x)" % (n, seed), "pixels": pixels,
                 "refused": str(e)}
+    frames = b.stats()["frames"]
     fine = {"config": cia.FINE_CONFIG_NAMES[0], "tuned": False}
     if tune:
         from canvas_ity_amd import tuning
@@ -307,7 +322,8 @@ def bench_batch(cia, device, n, seed=0xC4, reps=3, tune=False):
         c.close()
     return {"workload": "batch mix: %d canvases log-uniform 64..1024, 10..400 paths (seed %#x)"
                         % (n, seed),
-            "pixels": pixels, "batched_canvases_per_s": round(n / batched, 2),
+            "pixels": pixels, "frames": frames,
+            "batched_canvases_per_s": round(n / batched, 2),
             "plain_canvases_per_s": round(n / plain, 2), "ratio": round(plain / batched, 3),
             "batched_mpix_per_s": round(pixels / batched / 1e6, 2),
             "batched_with_srgb8_to_host_canvases_per_s": round(n / with_host, 2),

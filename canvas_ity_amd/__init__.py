@@ -69,6 +69,7 @@ SIGNATURES = {
     "cih_batch_image": (_c.c_int, [_P, _c.c_int, _P]),
     "cih_batch_linear": (_c.c_int, [_P, _c.c_int, _P]),
     "cih_batch_stats": (_c.c_int, [_P, _c.POINTER(_c.c_uint64)]),
+    "cih_batch_frame": (_c.c_int, [_P, _c.c_int, _c.POINTER(_c.c_uint64)]),
     "cih_batch_configure": (_c.c_int, [_P, _c.c_int]),
     "cih_batch_tune": (_c.c_int, [_P, _c.c_int, _c.POINTER(_c.c_double),
                                   _c.POINTER(_c.c_int)]),
@@ -396,6 +397,16 @@ class Batch:
         out = (_c.c_uint64 * 4)()
         _check(library().cih_batch_stats(self.handle, out), "cih_batch_stats")
         return {"canvases": out[0], "draws": out[1], "frames": out[2], "pixels": out[3]}
+
+    def frames(self):
+        """Each frame's first draw, draws, canvases and canvas pixels."""
+        out = []
+        info = (_c.c_uint64 * 4)()
+        for k in range(self.stats()["frames"]):
+            _check(library().cih_batch_frame(self.handle, k, info), "cih_batch_frame")
+            out.append({"first_draw": info[0], "draws": info[1], "canvases": info[2],
+                        "pixels": info[3]})
+        return out
 
     def configure(self, config):
         """Fine-stage launch shape, 0..FINE_CONFIGS-1 (FINE_CONFIG_NAMES)."""

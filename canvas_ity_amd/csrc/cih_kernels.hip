@@ -2563,12 +2563,15 @@ hipError_t launch_frame(const frame &F, int shadow_rows, int shadow_cols, bool a
         k_blur_cols<<<dim3(F.ndraws, cx > 0 ? cx : 1), 64, 0, stream>>>(F);
     MARK(7);
     if (F.nitems) {
-        // frames with masked draws always take the default 256-pixel item
+        // the host gives frames with masked draws the default configuration
+        // (256-pixel items, one workgroup each): the masked variant has only it
         unsigned grid = FINE_XCDS * ((F.nitems + FINE_XCDS - 1) / FINE_XCDS);
-        if (F.fine_persistent && !masked && grid > FINE_PERSISTENT_GRID)
+        if (F.fine_persistent && grid > FINE_PERSISTENT_GRID)
             grid = FINE_PERSISTENT_GRID;
-        int px = masked ? FINE_PX : F.segment / FINE_THREADS;
-        bool persistent = F.fine_persistent && !masked;
+        int px = F.segment / FINE_THREADS;
+        bool persistent = F.fine_persistent != 0;
+        if (masked && (px != FINE_PX || persistent))
+            return hipErrorInvalidValue; // cannot happen: run_kernels sees to it
         if (masked)
             k_fine<true, FINE_PX, false><<<grid, FINE_THREADS, 0, stream>>>(F);
         else if (px == 1 && persistent)

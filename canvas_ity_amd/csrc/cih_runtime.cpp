@@ -39,17 +39,21 @@ hipError_t launch_clear_batch(const void *canvases, const void *pixel_prefix, in
                               uint64_t total, hipStream_t stream);
 const int kFrameKernels = 8; // reset, geometry, runs, coalesce, shadow raster, blur rows/cols, fine
 
-// What one batch (and one server batch of several requests) may hold: the
-// largest multi-canvas frame that has run cleanly on the MI355X (76 canvases,
-// 4.98M pixels, 2,141 draws), rounded up.  Frames of 25,187 draws over 128
-// canvases (14.4M pixels) and larger batches have faulted the device, and the
-// cause has not been confirmed on the device.  Batches above these bounds are
-// refused with CIH_ERROR_BUDGET; the server runs a request above them as a
-// batch of its own (a single-canvas frame, like the plain path).
-const uint64_t kBatchPixelLimit = (uint64_t)5 << 20;
-const size_t kBatchDrawLimit = 2304;
-// Draws per frame of a batch or server batch (as for a single canvas).
+// What one frame over several canvases (of a batch or a server batch) may
+// hold: the largest multi-canvas frames that have run cleanly on the MI355X
+// (1,913 draws over 10 canvases; 1.9M pixels over 29 canvases), rounded up.
+// One frame of 25,187 draws over 128 canvases (14.4M pixels) has faulted the
+// device, and the cause has not been confirmed on the device.  A batch cuts its draws into frames within
+// these bounds; the server admits requests into a batch up to them and runs a
+// request above them as a batch of its own (a single-canvas frame, like the
+// plain path).
+const uint64_t kBatchPixelLimit = (uint64_t)2 << 20;
+const size_t kBatchDrawLimit = 2048;
+// Draws per frame over one canvas (the plain path's frames).
 const size_t kFrameDrawLimit = 8192;
+// Canvas pixels one batch may hold: 16 bytes of fp32 bitmap and 4 of sRGB8
+// each, 20 GiB of HBM at the bound.
+const uint64_t kBatchTotalPixelLimit = (uint64_t)1 << 30;
 
 // Bytes of draw packets one canvas program, batch or server batch may hold:
 // draw records address the payload with 32-bit offsets.
@@ -526,6 +530,8 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
                 const canvas_desc *canvases_dev, size_t ncanvases, const uint32_t *items_dev,
                 const frame_shape &shape, bool check, fine_config fine = fine_config()) {
     int s;
+    if (shape.masked)
+        fine = fine_config(); // k_fine's masked variant runs 256-pixel items only
     if (check) {
         if ((s = L.state.reserve(state_bytes(n))) ||
             (s = L.counters.reserve(8 * COUNTER_COUNT)) ||
@@ -1376,6 +1382,7 @@ struct cih_batch {
     std::vector<draw_desc> descs;
     struct frame_rec {
         size_t first_draw, ndraws, canvases_at, ncanvases, items_at;
+        uint64_t pixels; // of the canvases the frame draws on
         frame_shape shape;
     };
     std::vector<frame_rec> frames;
@@ -1489,11 +1496,11 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
                 b->pixel_prefix[(size_t)i] + (uint64_t)r.width * (uint64_t)r.height;
         }
         uint64_t total = b->pixel_prefix.back();
-        if (total > kBatchPixelLimit) {
+        if (total > kBatchTotalPixelLimit) {
             destroy_batch(b);
             return fail(CIH_ERROR_BUDGET, "a batch may hold at most " +
-                                              std::to_string(kBatchPixelLimit) +
-                                              " canvas pixels in this build (it has " +
+                                              std::to_string(kBatchTotalPixelLimit) +
+                                              " canvas pixels (it has " +
                                               std::to_string(total) + ")");
         }
         if ((s = b->bitmaps.reserve(16 * total)) || (s = b->srgb.reserve(4 * total)) ||
@@ -1570,18 +1577,13 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
             std::vector<unsigned char>().swap(v->pending);
             first_draw[(size_t)i + 1] = b->descs.size();
         }
-        if (b->descs.size() > kBatchDrawLimit) {
-            size_t draws = b->descs.size();
-            destroy_batch(b);
-            return fail(CIH_ERROR_BUDGET, "a batch may hold at most " +
-                                              std::to_string(kBatchDrawLimit) +
-                                              " draws in this build (it has " +
-                                              std::to_string(draws) + ")");
-        }
         // each canvas' clip masks (allocated while decoding)
         for (int i = 0; i < count; ++i)
             table[(size_t)i].masks = (uint64_t)(uintptr_t)b->views[(size_t)i]->mask_table.ptr;
-        // frames: consecutive draw ranges of bounded size; canvas lists per frame
+        // frames: consecutive draw ranges of bounded size; canvas lists per
+        // frame.  A frame over one canvas takes up to kFrameDrawLimit draws; one
+        // that spans several keeps to kBatchDrawLimit draws and kBatchPixelLimit
+        // canvas pixels (a first canvas above that runs in a frame of its own).
         const size_t max_draws = kFrameDrawLimit;
         const uint64_t shadow_budget = (uint64_t)1 << 30;
         std::vector<canvas_desc> frame_canvases;
@@ -1590,10 +1592,20 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
         size_t draw = 0;
         while (draw < b->descs.size()) {
             size_t last = draw;
-            uint64_t shadow = 0;
-            int current = -1;
+            uint64_t shadow = 0, pixels = 0;
+            int current = -1, first_canvas = b->descs[draw].canvas, last_canvas = -1;
             while (last < b->descs.size() && last - draw < max_draws) {
                 const draw_desc &dd = b->descs[last];
+                if (dd.canvas != first_canvas && last - draw >= kBatchDrawLimit)
+                    break;
+                if (dd.canvas != last_canvas) {
+                    uint64_t px = b->pixel_prefix[(size_t)dd.canvas + 1] -
+                                  b->pixel_prefix[(size_t)dd.canvas];
+                    if (last > draw && pixels + px > kBatchPixelLimit)
+                        break;
+                    pixels += px;
+                    last_canvas = dd.canvas;
+                }
                 if (frame_cut(dd, dd.canvas, used, current) && last > draw)
                     break;
                 uint64_t need = 0;
@@ -1611,6 +1623,7 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
             f.first_draw = draw;
             f.ndraws = last - draw;
             f.canvases_at = frame_canvases.size();
+            f.pixels = pixels;
             int c0 = b->descs[draw].canvas, c1 = b->descs[last - 1].canvas;
             for (int c = c0; c <= c1; ++c) {
                 size_t lo = std::max(first_draw[(size_t)c], draw);
@@ -1859,6 +1872,19 @@ int cih_batch_stats(cih_batch *b, uint64_t *stats) {
     stats[1] = b->descs.size();
     stats[2] = b->frames.size();
     stats[3] = b->pixel_prefix.back();
+    return CIH_OK;
+}
+
+int cih_batch_frame(cih_batch *b, int index, uint64_t *info) {
+    if (!b || !info)
+        return fail(CIH_ERROR_INVALID, "NULL argument");
+    if (index < 0 || (size_t)index >= b->frames.size())
+        return fail(CIH_ERROR_INVALID, "frame index out of range");
+    const cih_batch::frame_rec &f = b->frames[(size_t)index];
+    info[0] = f.first_draw;
+    info[1] = f.ndraws;
+    info[2] = f.ncanvases;
+    info[3] = f.pixels;
     return CIH_OK;
 }
 

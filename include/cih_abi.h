@@ -254,10 +254,12 @@ CIH_API int cih_text_outline(const unsigned char *font, size_t font_bytes, float
  * it to pinned host memory and waits.  No reference counterpart: the
  * reference renders one canvas per object (canvas_ity.hpp:194). */
 typedef struct cih_batch cih_batch;
-/* A batch holds at most 5M (5 * 2^20) canvas pixels and 2304 draws in this
- * build (CIH_ERROR_BUDGET beyond either).  A server batch admits requests up
- * to the same totals; a single request above them runs as a batch of its
- * own. */
+/* A batch holds at most 2^30 canvas pixels (CIH_ERROR_BUDGET beyond) and any
+ * number of draws.  It cuts its draws, in order, into frames: a frame over
+ * several canvases holds at most 2048 draws and 2M (2 * 2^20) canvas pixels,
+ * one over a single canvas at most 8192 draws (as the plain path's frames).
+ * A server batch admits requests up to 2048 draws and 2M pixels; a single
+ * request above them runs as a batch of its own. */
 CIH_API int cih_batch_create(cih_device *device, int count, const void *const *streams,
                              const size_t *bytes, cih_batch **batch);
 CIH_API int cih_batch_execute(cih_batch *batch, int to_host);
@@ -268,6 +270,9 @@ CIH_API int cih_batch_image(cih_batch *batch, int index, unsigned char *image);
 CIH_API int cih_batch_linear(cih_batch *batch, int index, float *rgba);
 /* stats: canvases, draws, frames, pixels. */
 CIH_API int cih_batch_stats(cih_batch *batch, uint64_t *stats);
+/* info of frame `index` (0 .. frames - 1): first draw, draws, canvases, and
+ * the pixels of those canvases (test seam; no reference counterpart). */
+CIH_API int cih_batch_frame(cih_batch *batch, int index, uint64_t *info);
 /* Fine-stage launch shape of a batch (no reference counterpart): config
  * 2 * w + p, w = 0, 1, 2 for work items of 256, 128 or 64 pixels of a canvas
  * row, p = 1 for one persistent grid of waves draining per-XCD item queues

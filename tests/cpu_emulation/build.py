@@ -256,17 +256,17 @@ def emulated_runtime(src):
     # EMU_BATCH_DRAWS / EMU_BATCH_PIXELS raise the batch bounds in the
     # emulation only, so batches that are refused on the device can be run
     # on the CPU
-    sub("const uint64_t kBatchPixelLimit = (uint64_t)5 << 20;",
+    sub("const uint64_t kBatchPixelLimit = (uint64_t)2 << 20;",
         "const uint64_t kBatchPixelLimit = getenv(\"EMU_BATCH_PIXELS\") ? "
-        "strtoull(getenv(\"EMU_BATCH_PIXELS\"), nullptr, 0) : (uint64_t)5 << 20;")
+        "strtoull(getenv(\"EMU_BATCH_PIXELS\"), nullptr, 0) : (uint64_t)2 << 20;")
     # EMU_PAYLOAD_LIMIT lowers the 4 GiB payload limit, so its refusal can be
     # checked without gigabytes of packets
     sub("const uint64_t kPayloadLimit = 0xffffffffull;",
         "const uint64_t kPayloadLimit = getenv(\"EMU_PAYLOAD_LIMIT\") ? "
         "strtoull(getenv(\"EMU_PAYLOAD_LIMIT\"), nullptr, 0) : 0xffffffffull;")
-    sub("const size_t kBatchDrawLimit = 2304;",
+    sub("const size_t kBatchDrawLimit = 2048;",
         "const size_t kBatchDrawLimit = getenv(\"EMU_BATCH_DRAWS\") ? "
-        "strtoull(getenv(\"EMU_BATCH_DRAWS\"), nullptr, 0) : 2304;")
+        "strtoull(getenv(\"EMU_BATCH_DRAWS\"), nullptr, 0) : 2048;")
     sub("std::max(need, bytes + bytes / 2)", "need")
     sub("attempt < 8;", "attempt < 64;")
     sub("    L.last_ndraws = n;\n", SAVE_CANVASES)

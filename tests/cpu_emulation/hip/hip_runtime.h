@@ -9,7 +9,7 @@
 #define __HIP_DEVICE_COMPILE__ 1
 
 typedef int hipError_t;
-enum { hipSuccess = 0, hipErrorNotReady = 600, hipErrorNotSupported = 801, hipErrorOutOfMemory = 2 };
+enum { hipSuccess = 0, hipErrorInvalidValue = 1, hipErrorOutOfMemory = 2, hipErrorNotReady = 600, hipErrorNotSupported = 801 };
 typedef struct emu_stream *hipStream_t;
 typedef struct emu_event *hipEvent_t;
 typedef struct emu_graph *hipGraph_t;

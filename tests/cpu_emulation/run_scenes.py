@@ -71,19 +71,82 @@ def batch_small(device):
     b.close()
 
 
-def batch_limits(device):
-    """Batches above the draw or pixel bound are refused before any kernel
-    runs: the 128-canvas batch of the batch mix (14.4M pixels, 25,187 draws),
-    and three 64x64 canvases of 2000 paths each (over 2304 draws)."""
+FRAME_DRAWS = 8192  # draws of a frame over one canvas
+
+
+def frame_bounds():
+    """The multi-canvas frame bounds the library was built with (the
+    emulation lowers them through EMU_BATCH_DRAWS / EMU_BATCH_PIXELS)."""
+    return (int(os.environ.get("EMU_BATCH_DRAWS", "2048")),
+            int(os.environ.get("EMU_BATCH_PIXELS", str(2 << 20))))
+
+
+def check_frames(b):
+    """The batch's frames cover its draws in order, each within its bound;
+    returns them."""
+    draws_bound, pixels_bound = frame_bounds()
+    stats, frames = b.stats(), b.frames()
+    assert len(frames) == stats["frames"] and frames, stats
+    at = 0
+    for f in frames:
+        assert f["first_draw"] == at and f["draws"] > 0 and f["canvases"] > 0, (f, at)
+        at += f["draws"]
+        if f["canvases"] == 1:
+            assert f["draws"] <= FRAME_DRAWS, f
+        else:
+            assert f["draws"] <= draws_bound and f["pixels"] <= pixels_bound, f
+    assert at == stats["draws"], (at, stats)
+    # frames are filled greedily: fewer than twice the frames either bound asks
+    pixels = sum(f["pixels"] for f in frames)
+    need = -(-stats["draws"] // draws_bound) + -(-pixels // pixels_bound)
+    assert len(frames) <= 2 * need + 2, (len(frames), need)
+    return frames
+
+
+def batch_frames(device):
+    """A batch of any draw count is cut into frames within the multi-canvas
+    bounds: the 128-canvas batch of the batch mix (14.4M pixels, 25,187
+    draws; as one frame it faulted the device) and three 64x64 canvases of
+    2000 paths each.  A batch above 2^30 canvas pixels is refused before
+    anything is allocated."""
     big = [cia.scene(4, cia.scene_seed(0xC4, i)) for i in range(128)]
     many = [cia.scene(4, cia.scene_seed(0x0C, i), 64, 64, 2000) for i in range(3)]
-    for streams, what in ((big, "canvas pixels"), (many, "draws")):
-        try:
-            cia.Batch(streams, device)
-        except cia.CihError as e:
-            assert "budget" in str(e) and what in str(e), e
-        else:
-            raise AssertionError("a batch above the bound was accepted")
+    for streams in (big, many):
+        b = cia.Batch(streams, device)
+        frames = check_frames(b)
+        assert len(frames) > 1, frames
+        b.close()
+    from canvas_ity_amd.stream import Stream
+    huge = [Stream(32768, 32768).tobytes() for _ in range(2)]
+    try:
+        cia.Batch(huge, device)
+    except cia.CihError as e:
+        assert "budget" in str(e) and "canvas pixels" in str(e), e
+    else:
+        raise AssertionError("a batch above 2^30 pixels was accepted")
+
+
+def batch_frames_render(device):
+    """With the frame bounds lowered (EMU_BATCH_DRAWS=50,
+    EMU_BATCH_PIXELS=10000) eight small canvases of the batch mix's paths are
+    cut into many frames, some of them across a canvas: every canvas renders
+    as it does alone, on the checked run and on the graph replay."""
+    assert frame_bounds() == (50, 10000)
+    sizes = [(48, 32, 20), (64, 48, 40), (32, 32, 12), (80, 60, 60), (40, 50, 24),
+             (56, 36, 30), (32, 64, 45), (72, 48, 28)]
+    streams = [cia.scene(4, cia.scene_seed(0x5F, i), w, h, n)
+               for i, (w, h, n) in enumerate(sizes)]
+    alone = [cia.render(s, device)[1] for s in streams]
+    b = cia.Batch(streams, device)
+    frames = check_frames(b)
+    assert len(frames) >= 6, frames
+    assert any(f["canvases"] > 1 for f in frames), frames
+    assert sum(f["canvases"] for f in frames) > len(streams), frames  # a canvas split
+    for _ in range(2):  # the checked run, then the captured graph
+        b.execute(to_host=True)
+        for i in range(len(streams)):
+            assert np.array_equal(alone[i].view(np.uint32), b.linear(i).view(np.uint32)), i
+    b.close()
 
 
 def rectangles(n, seed):
@@ -138,17 +201,23 @@ def fine_configs(device):
     import tempfile
     from canvas_ity_amd import tuning
     all_streams = streams()
-    names = ["plain_small", "large_small", "effects_small", "text_scene_1"]
-    batch_streams = [all_streams[n] for n in names]
-    b = cia.Batch(batch_streams, device)
-    b.execute(to_host=True)
-    ref = [b.linear(i).copy() for i in range(len(names))]
-    for config in range(1, cia.FINE_CONFIGS):
-        b.configure(config)
+    # the second batch is one frame with masked draws (clip_nested): it keeps
+    # the default configuration whatever the batch is configured to
+    groups = (["plain_small", "large_small", "effects_small", "text_scene_1"],
+              ["clip_nested", "plain_small"])
+    for names in groups[::-1]:
+        batch_streams = [all_streams[n] for n in names]
+        b = cia.Batch(batch_streams, device)
         b.execute(to_host=True)
-        for i, n in enumerate(names):
-            got = b.linear(i)
-            assert np.array_equal(ref[i].view(np.uint32), got.view(np.uint32)), (config, n)
+        ref = [b.linear(i).copy() for i in range(len(names))]
+        for config in range(1, cia.FINE_CONFIGS):
+            b.configure(config)
+            b.execute(to_host=True)
+            for i, n in enumerate(names):
+                got = b.linear(i)
+                assert np.array_equal(ref[i].view(np.uint32), got.view(np.uint32)), (config, n)
+        if names is not groups[0]:
+            b.close()
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "tune.json")
         config, key, tuned = tuning.configure(b, batch_streams, path=path, reps=1)
@@ -216,7 +285,8 @@ def batch_mix(device, n):
 
 
 MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small,
-         "batch_limits": batch_limits, "payload_limit": payload_limit,
+         "batch_frames": batch_frames, "batch_frames_render": batch_frames_render,
+         "payload_limit": payload_limit,
          "fine_configs": fine_configs, "canvas_execute": canvas_execute,
          "reference_suite_batch": reference_suite_batch}
 

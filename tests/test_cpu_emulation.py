@@ -100,10 +100,15 @@ def test_payloads_above_the_offset_range_are_refused(library):
     run(library, ["payload_limit"], extra={"EMU_PAYLOAD_LIMIT": "20000"})
 
 
-def test_batches_above_the_bounds_are_refused(library):
-    # the batch regime that faulted the device (many canvases, tens of
-    # thousands of draws in one frame) is refused with CIH_ERROR_BUDGET
-    run(library, ["batch_limits"])
+def test_batches_are_cut_into_bounded_frames(library):
+    # the batch regime that faulted the device (tens of thousands of draws
+    # over many canvases in one frame) is cut into frames within the
+    # multi-canvas bounds; above 2^30 pixels a batch is refused with
+    # CIH_ERROR_BUDGET.  With the bounds lowered, a batch of many frames,
+    # some across a canvas, renders every canvas as it renders alone.
+    run(library, ["batch_frames"])
+    run(library, ["batch_frames_render"], threads=4,
+        extra={"EMU_BATCH_DRAWS": "50", "EMU_BATCH_PIXELS": "10000"})
 
 
 def test_strict_checks_with_concurrent_workgroups(library):

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import canvas_ity_amd as cia
+import scenes
 
 pytestmark = pytest.mark.gpu
 
@@ -31,4 +32,20 @@ def test_fine_configurations_are_bit_identical(device):
     b.execute(to_host=True)
     for i in range(len(streams)):
         assert np.array_equal(ref[i].view(np.uint32), b.linear(i).view(np.uint32)), i
+    b.close()
+
+
+def test_masked_frames_keep_the_default_configuration(device):
+    # a frame with draws under a clip mask runs 256-pixel items, one
+    # workgroup each, whatever the batch is configured to
+    clips = scenes.clip_scenes()
+    streams = [clips["nested"], cia.scene(1, 0xC1, 72, 56, 10, 0), clips["op_source_in"]]
+    b = cia.Batch(streams, device)
+    b.execute(to_host=True)
+    ref = [b.linear(i).copy() for i in range(len(streams))]
+    for config in range(1, cia.FINE_CONFIGS):
+        b.configure(config)
+        b.execute(to_host=True)
+        for i in range(len(streams)):
+            assert np.array_equal(ref[i].view(np.uint32), b.linear(i).view(np.uint32)), (config, i)
     b.close()
