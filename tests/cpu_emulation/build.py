@@ -148,21 +148,36 @@ def emulated_kernels(src):
             p += 1
         args = src[e + 4:p]
         # the name in parentheses: template arguments may hold commas
-        out += src[i:k] + "EMU_LAUNCH((%s), (%s), (%s)%s)" % (
-            name, cfg[0], cfg[1], (", " + args) if args.strip() else "")
+        stream = cfg[3] if len(cfg) > 3 else "0"
+        out += src[i:k] + "EMU_LAUNCH((%s), (%s), (%s), (%s)%s)" % (
+            name, cfg[0], cfg[1], stream, (", " + args) if args.strip() else "")
         i = p + 1
+    # grid-stride kernels on smaller fixed grids (EMU_FULL_GRIDS keeps the
+    # device's 4096 workgroups): the same code with more loop iterations per
+    # workgroup, at a fraction of the emulation's cost per workgroup; the
+    # persistent fine grid stays a multiple of the 8 XCD queues
+    for old, small in (("#define COALESCE_GRID (256 * 16)", 256),
+                       ("#define FINE_PERSISTENT_GRID (256 * 16)", 64)):
+        assert out.count(old) == 1, old
+        out = out.replace(old, old.replace("(256 * 16)", "emu_grid(256 * 16, %d)" % small))
     out = _shared_regions(out)
     out = _redzone_arena_alloc(out)
     out = re.sub(r'asm volatile\("s_waitcnt lgkmcnt\(0\)" ::: "memory"\);',
                  "emu::wave_barrier(__LINE__);", out)
     assert "asm" not in re.sub(r"//.*", "", out), "unhandled inline assembly"
     prelude = (
-        "#include <tuple>\n#include <utility>\n#include \"hip/hip_runtime.h\"\n"
+        "#include <cstdlib>\n#include <tuple>\n#include <utility>\n"
+        "#include \"hip/hip_runtime.h\"\n"
+        "static unsigned emu_grid(unsigned full, unsigned small) {\n"
+        "    static const bool device_grids = getenv(\"EMU_FULL_GRIDS\") != nullptr;\n"
+        "    return device_grids ? full : small;\n}\n"
         "template <class F, class T> static void emu_body(void *c) {\n"
         "    auto *p = (std::pair<F, T> *)c;\n    std::apply(p->first, p->second);\n}\n"
-        "#define EMU_LAUNCH(k, g, b, ...) do { auto emu_p = std::make_pair(&k, "
-        "std::make_tuple(__VA_ARGS__)); emu::run_kernel(#k, dim3(g), dim3(b), "
-        "&emu_body<decltype(emu_p.first), decltype(emu_p.second)>, &emu_p); } while (0)\n")
+        "#define EMU_LAUNCH(k, g, b, s, ...) do { auto emu_f = &k; "
+        "auto emu_t = std::make_tuple(__VA_ARGS__); dim3 emu_g(g), emu_b(b); "
+        "emu::launch(s, [=]() { auto emu_p = std::make_pair(emu_f, emu_t); "
+        "emu::run_kernel(#k, emu_g, emu_b, "
+        "&emu_body<decltype(emu_p.first), decltype(emu_p.second)>, &emu_p); }); } while (0)\n")
     return prelude + out
 
 
@@ -198,6 +213,8 @@ SAVE_CANVASES = """    L.last_ndraws = n;
     uint64_t keep[4] = {0, 0, 0, 0};
     bool regrow = false;
     std::vector<std::vector<unsigned char>> saved(ncanvases);
+    if (check) // the bitmaps as earlier work on the lane leaves them
+        HIP_TRY(hipStreamSynchronize(L.stream));
     if (check)
         for (size_t c = 0; c < ncanvases; ++c) {
             const canvas_desc &cd = canvases_dev[c];
@@ -207,11 +224,17 @@ SAVE_CANVASES = """    L.last_ndraws = n;
 """
 
 
-UNPOISON_ARENAS = """        // emulation: arena red zones of an earlier frame are cleared
+UNPOISON_ARENAS = """        // emulation: arena red zones of an earlier frame are cleared, in
+        // stream order (its kernels may still be queued)
         if (emu::redzones()) {
             dbuf *arenas[] = {&L.points, &L.words, &L.runs, &L.steps, &L.shadow};
+            std::vector<std::pair<void *, size_t>> zones;
             for (dbuf *a : arenas)
-                emu::unpoison(a->ptr, a->bytes);
+                zones.emplace_back(a->ptr, a->bytes);
+            emu::launch(L.stream, [zones]() {
+                for (const auto &z : zones)
+                    emu::unpoison(z.first, z.second);
+            });
         }
         HIP_TRY(launch_frame(F,"""
 

@@ -13,6 +13,11 @@
 #include <condition_variable>
 #include <thread>
 #include <unordered_map>
+#include <chrono>
+#include <string>
+#include <deque>
+#include <map>
+#include <set>
 #include <sanitizer/asan_interface.h>
 #include <sanitizer/common_interface_defs.h>
 
@@ -21,10 +26,10 @@ namespace emu {
 // A lane is a fiber that lives for the whole process: started once (the only
 // ucontext switch, which costs signal-mask system calls), it runs the kernel
 // body for a workgroup, parks at `idle`, and is restarted there for the next
-// workgroup with a plain _longjmp.
+// workgroup with a plain jump (jump_to).
 struct fiber {
     ucontext_t uctx;
-    jmp_buf jb, idle;
+    void *jb[5], *idle[5]; // __builtin_setjmp buffers
     bool alive = false;   // started once; parked at `idle` between workgroups
     bool started = false, done = false;
     idx3 tid;
@@ -48,7 +53,14 @@ struct rendezvous {
 static const size_t kStack = 512 * 1024;
 static thread_local fiber *fibers = nullptr; // 1024 lanes, allocated per thread
 static thread_local fiber *cur = nullptr;
-static thread_local jmp_buf sched_jb;
+static thread_local void *sched_jb[5];
+
+// Context switches between the scheduler and the lanes' stacks use GCC's
+// __builtin_setjmp / __builtin_longjmp: AddressSanitizer intercepts the
+// library longjmp and unpoisons the current stack from there on, which here
+// is a live lane stack, not frames being unwound (and costs a shadow fill
+// per switch).  The longjmp sits in a function of its own, as GCC requires.
+[[gnu::noinline]] static void jump_to(void **buf) { __builtin_longjmp(buf, 1); }
 static thread_local void *sched_fake = nullptr;
 static thread_local const void *sched_bottom = nullptr;
 static thread_local size_t sched_size = 0;
@@ -108,9 +120,9 @@ void fail(const char *msg) {
 
 static void to_scheduler() {
     fiber *f = cur;
-    if (!_setjmp(f->jb)) {
+    if (!__builtin_setjmp(f->jb)) {
         __sanitizer_start_switch_fiber(&f->fake, sched_bottom, sched_size);
-        _longjmp(sched_jb, 1);
+        jump_to(sched_jb);
     }
     __sanitizer_finish_switch_fiber(f->fake, nullptr, nullptr);
 }
@@ -180,9 +192,9 @@ static void fiber_entry() {
         block.live--;
         if (block.arrived && block.arrived == block.live)
             complete(block);
-        if (!_setjmp(f->idle)) { // park until the next workgroup
+        if (!__builtin_setjmp(f->idle)) { // park until the next workgroup
             __sanitizer_start_switch_fiber(&f->fake, sched_bottom, sched_size);
-            _longjmp(sched_jb, 1);
+            jump_to(sched_jb);
         }
         __sanitizer_finish_switch_fiber(cur->fake, &sched_bottom, &sched_size);
     }
@@ -224,11 +236,11 @@ static void run_block(unsigned bx, unsigned by, unsigned bz) {
             if (f.done)
                 continue;
             cur = &f;
-            if (!_setjmp(sched_jb)) {
+            if (!__builtin_setjmp(sched_jb)) {
                 if (!f.started && f.alive) { // restart a parked lane
                     f.started = true;
                     __sanitizer_start_switch_fiber(&sched_fake, f.stack, kStack);
-                    _longjmp(f.idle, 1);
+                    jump_to(f.idle);
                 } else if (!f.started) {
                     f.started = true;
                     f.alive = true;
@@ -242,7 +254,7 @@ static void run_block(unsigned bx, unsigned by, unsigned bz) {
                     swapcontext(&dummy, &f.uctx);
                 } else {
                     __sanitizer_start_switch_fiber(&sched_fake, f.stack, kStack);
-                    _longjmp(f.jb, 1);
+                    jump_to(f.jb);
                 }
             }
             __sanitizer_finish_switch_fiber(sched_fake, nullptr, nullptr);
@@ -330,8 +342,27 @@ static int emu_threads() {
     return n;
 }
 
+// EMU_PROFILE: launches, workgroups and seconds per kernel, printed at exit.
+struct kernel_profile {
+    unsigned long long launches = 0, workgroups = 0;
+    double seconds = 0;
+};
+static std::map<std::string, kernel_profile> *profiles;
+
+static void print_profiles() {
+    for (const auto &p : *profiles)
+        fprintf(stderr, "[profile] %-40s %8llu launches %12llu workgroups %9.2f s\n",
+                p.first.c_str(), p.second.launches, p.second.workgroups, p.second.seconds);
+}
+
 void run_kernel(const char *name, dim3 grid, dim3 blk, void (*body)(void *), void *ctx) {
     std::lock_guard<std::recursive_mutex> lock(run_mutex);
+    static const bool profiling = env_on("EMU_PROFILE");
+    auto started = std::chrono::steady_clock::now();
+    if (profiling && !profiles) {
+        profiles = new std::map<std::string, kernel_profile>;
+        atexit(print_profiles);
+    }
     g_name = name;
     g_body = body;
     g_ctx = ctx;
@@ -367,14 +398,94 @@ void run_kernel(const char *name, dim3 grid, dim3 blk, void (*body)(void *), voi
         drain();
     }
     kernels_run++;
+    if (profiling) {
+        kernel_profile &p = (*profiles)[name];
+        p.launches++;
+        p.workgroups += total;
+        p.seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - started).count();
+    }
 }
 
 } // namespace emu
 
 // ---- runtime --------------------------------------------------------------------
-struct emu_stream { int id; };
-struct emu_event { int id; };
+// EMU_ASYNC_STREAMS: a stream's kernels, device-to-device copies, copies from
+// or to pinned host memory and fills are queued and run only when the host
+// waits for them: hipStreamSynchronize, hipEventSynchronize, hipEventQuery of
+// an event recorded after them (the first query of each recording reports
+// hipErrorNotReady and runs nothing), hipStreamDestroy, or a device-wide wait
+// (hipFree and hipHostFree synchronise the device, as HIP documents).  A copy
+// between pageable host memory and the device first runs the stream's earlier
+// work and then copies at once, as HIP documents for unpinned memory.  Host
+// code that reads a result before waiting for it, or rewrites pinned staging
+// that a queued copy has yet to read, then gets wrong pixels; a buffer freed
+// while queued work still uses it is a use-after-free.  Without the mode,
+// work runs when it is issued.
+struct emu_stream {
+    std::mutex m;
+    std::deque<std::function<void()>> q;
+    uint64_t queued = 0, done = 0;
+};
+struct emu_event {
+    emu_stream *stream = nullptr;
+    uint64_t at = 0; // the stream's queued count when recorded
+    bool queried = false;
+};
 static int g_err = hipSuccess;
+static std::mutex streams_mutex;
+static std::set<emu_stream *> live_streams;
+static std::recursive_mutex device_mutex; // queued work runs one op at a time
+static std::mutex pinned_mutex;
+static std::map<uintptr_t, size_t> pinned; // hipHostMalloc blocks
+
+static bool async_streams() {
+    static const bool on = emu::env_on("EMU_ASYNC_STREAMS");
+    return on;
+}
+
+static bool is_pinned(const void *p) {
+    std::lock_guard<std::mutex> lock(pinned_mutex);
+    uintptr_t a = (uintptr_t)p;
+    auto it = pinned.upper_bound(a);
+    if (it == pinned.begin())
+        return false;
+    --it;
+    return a < it->first + it->second;
+}
+
+// Runs stream st's queued work until `upto` items of it have run.
+static void drain_stream(emu_stream *st, uint64_t upto) {
+    std::lock_guard<std::recursive_mutex> device(device_mutex);
+    for (;;) {
+        std::function<void()> op;
+        {
+            std::lock_guard<std::mutex> lock(st->m);
+            if (st->done >= upto || st->q.empty())
+                return;
+            op = std::move(st->q.front());
+            st->q.pop_front();
+        }
+        op();
+        std::lock_guard<std::mutex> lock(st->m);
+        st->done++;
+    }
+}
+
+static void drain_all() {
+    std::lock_guard<std::mutex> lock(streams_mutex);
+    for (emu_stream *st : live_streams)
+        drain_stream(st, UINT64_MAX);
+}
+
+void emu::launch(hipStream_t st, std::function<void()> op) {
+    if (!async_streams() || !st) {
+        op();
+        return;
+    }
+    std::lock_guard<std::mutex> lock(st->m);
+    st->q.push_back(std::move(op));
+    st->queued++;
+}
 
 // EMU_STRICT_STREAMS: host-to-device copies on the NULL stream are held back
 // until the NULL stream or the device is synchronised (or another NULL-stream
@@ -409,12 +520,30 @@ hipError_t hipMalloc(void **p, size_t n) {
         memset(*p, 0xA5, n ? n : 1);
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
-hipError_t hipFree(void *p) { land_held_copies(); free(p); return hipSuccess; }
+hipError_t hipFree(void *p) {
+    drain_all();
+    land_held_copies();
+    free(p);
+    return hipSuccess;
+}
 hipError_t hipHostMalloc(void **p, size_t n, unsigned) {
     *p = malloc(n ? n : 1);
+    if (*p) {
+        std::lock_guard<std::mutex> lock(pinned_mutex);
+        pinned[(uintptr_t)*p] = n ? n : 1;
+    }
     return *p ? hipSuccess : hipErrorOutOfMemory;
 }
-hipError_t hipHostFree(void *p) { land_held_copies(); free(p); return hipSuccess; }
+hipError_t hipHostFree(void *p) {
+    drain_all();
+    land_held_copies();
+    {
+        std::lock_guard<std::mutex> lock(pinned_mutex);
+        pinned.erase((uintptr_t)p);
+    }
+    free(p);
+    return hipSuccess;
+}
 hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k) {
     land_held_copies(); // the NULL stream runs its copies in order
     if (!n)
@@ -431,29 +560,79 @@ hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind k) {
 hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t st) {
     if (!st)
         return hipMemcpy(d, s, n, k);
-    if (n)
-        memmove(d, s, n); // stream-ordered: kernels of the stream run later
+    if (!n)
+        return hipSuccess;
+    if (async_streams()) {
+        bool queued = k == hipMemcpyDeviceToDevice ||
+                      (k == hipMemcpyHostToDevice && is_pinned(s)) ||
+                      (k == hipMemcpyDeviceToHost && is_pinned(d));
+        if (queued) { // a pinned source is read when the copy runs
+            emu::launch(st, [d, s, n]() { memmove(d, s, n); });
+            return hipSuccess;
+        }
+        drain_stream(st, UINT64_MAX); // pageable memory: a synchronous copy
+    }
+    memmove(d, s, n);
     return hipSuccess;
 }
-hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t) { memset(d, v, n); return hipSuccess; }
+hipError_t hipMemsetAsync(void *d, int v, size_t n, hipStream_t st) {
+    emu::launch(st, [d, v, n]() { memset(d, v, n); });
+    return hipSuccess;
+}
 hipError_t hipStreamSynchronize(hipStream_t st) {
     if (!st)
         land_held_copies();
+    else
+        drain_stream(st, UINT64_MAX);
     return hipSuccess;
 }
-hipError_t hipStreamCreate(hipStream_t *s) { *s = new emu_stream{1}; return hipSuccess; }
+hipError_t hipStreamCreate(hipStream_t *s) {
+    *s = new emu_stream;
+    std::lock_guard<std::mutex> lock(streams_mutex);
+    live_streams.insert(*s);
+    return hipSuccess;
+}
 hipError_t hipStreamCreateWithFlags(hipStream_t *s, unsigned) { return hipStreamCreate(s); }
-hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
+hipError_t hipStreamDestroy(hipStream_t s) {
+    drain_stream(s, UINT64_MAX);
+    {
+        std::lock_guard<std::mutex> lock(streams_mutex);
+        live_streams.erase(s);
+    }
+    delete s;
+    return hipSuccess;
+}
 hipError_t hipGetLastError() { int e = g_err; g_err = hipSuccess; return e; }
 const char *hipGetErrorString(hipError_t) { return "emulated error"; }
 hipError_t hipSetDevice(int) { return hipSuccess; }
 hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
-hipError_t hipEventCreate(hipEvent_t *e) { *e = new emu_event{1}; return hipSuccess; }
+hipError_t hipEventCreate(hipEvent_t *e) { *e = new emu_event; return hipSuccess; }
 hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
 hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
-hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
-hipError_t hipEventQuery(hipEvent_t) { return hipSuccess; }
-hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventRecord(hipEvent_t e, hipStream_t st) {
+    e->stream = st;
+    e->queried = false;
+    if (st) {
+        std::lock_guard<std::mutex> lock(st->m);
+        e->at = st->queued;
+    }
+    return hipSuccess;
+}
+hipError_t hipEventQuery(hipEvent_t e) {
+    if (!async_streams() || !e->stream)
+        return hipSuccess;
+    if (!e->queried) {
+        e->queried = true;
+        return hipErrorNotReady;
+    }
+    drain_stream(e->stream, e->at);
+    return hipSuccess;
+}
+hipError_t hipEventSynchronize(hipEvent_t e) {
+    if (e->stream)
+        drain_stream(e->stream, e->at);
+    return hipSuccess;
+}
 hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.0f; return hipSuccess; }
 hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorNotSupported; }
 hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *) { return hipErrorNotSupported; }

@@ -5,6 +5,7 @@
 #include <stdint.h>
 #include <string.h>
 #include <math.h>
+#include <functional>
 
 #define __HIP_DEVICE_COMPILE__ 1
 
@@ -78,6 +79,9 @@ void block_barrier(int site);
 void wave_barrier(int site);
 void fail(const char *msg);
 void run_kernel(const char *name, dim3 grid, dim3 block, void (*body)(void *), void *ctx);
+// stream-ordered work: run at once, or with EMU_ASYNC_STREAMS queued on the
+// stream until the host waits for it
+void launch(hipStream_t stream, std::function<void()> op);
 // stricter checks (see tests/cpu_emulation/build.py): EMU_POISON refills each
 // __shared__ object at every workgroup's start; EMU_REDZONES puts a poisoned
 // red zone after each arena sub-allocation

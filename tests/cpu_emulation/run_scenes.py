@@ -284,7 +284,46 @@ def batch_mix(device, n):
     b.close()
 
 
-MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small,
+def serve_burst(device):
+    """A closed-loop burst of serving-mix requests: the config-1 mix with
+    shadows on every third and the effects mix (composite operators, shadows,
+    draw_image), some streams sent twice, all submitted at once.  Admitted up
+    to 64 per batch within the frame bounds (which the run may lower with
+    EMU_BATCH_DRAWS / EMU_BATCH_PIXELS), with at most 3 per batch, and three
+    larger requests under an HBM budget that holds two of them, so admission
+    waits for memory.  Every result equals the request rendered alone."""
+    sizes = [(40, 32), (64, 48), (32, 56), (72, 40), (48, 48), (56, 64)]
+    streams = []
+    for i in range(8):
+        w, h = sizes[i % len(sizes)]
+        seed = cia.scene_seed(0xB5, i % 6)
+        if i % 4 == 1:
+            streams.append(cia.scene(3, seed, w, h, 5))  # one draw_image per 5 paths
+        else:
+            flags = cia.SCENE_SHADOWS if i % 3 == 0 else 0
+            streams.append(cia.scene(5, seed, w, h, 4 + 2 * (i % 4), flags))
+    large = [cia.scene(1, cia.scene_seed(0xB6, i), 256, 160, 4, 0) for i in range(3)]
+    alone = {id(s): cia.render(s, device)[0] for s in streams + large}
+    two = 2 * 16 * 256 * 160 + 4096
+    for batch, budget, max_batch in ((streams, 1 << 26, 64), (streams, 1 << 26, 3),
+                                     (large, two, 64)):
+        server = cia.Server(hbm_budget=budget, max_batch=max_batch, device=device)
+        ids = [server.submit(s) for s in batch]
+        got = {}
+        while len(got) < len(ids):
+            rid = server.wait(timeout_ms=600000)
+            assert rid is not None, "server stalled"
+            got[rid] = server.result(rid)[0]
+        stats = server.stats()
+        server.stop()
+        assert stats["served"] == len(batch), stats
+        if batch is large:
+            assert stats["batches"] >= 2 and stats["max_in_flight"] <= 2, stats
+        for s, rid in zip(batch, ids):
+            assert np.array_equal(alone[id(s)], got[rid]), (budget, max_batch, rid)
+
+
+MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small, "serve_burst": serve_burst,
          "batch_frames": batch_frames, "batch_frames_render": batch_frames_render,
          "payload_limit": payload_limit,
          "fine_configs": fine_configs, "canvas_execute": canvas_execute,
