@@ -75,9 +75,19 @@ def _load(kind):
         fn = lib.cih_oracle_is_point_in_path
         fn.restype = ctypes.c_int
         fn.argtypes = [_P, ctypes.c_float, ctypes.c_float]
+        for name in ("path_lines", "dash_lines", "lines_to_runs"):
+            fn = getattr(lib, "cih_oracle_" + name)
+            fn.restype = ctypes.c_long
+        lib.cih_oracle_path_lines.argtypes = [_P, ctypes.c_int, _P, ctypes.c_long, _P,
+                                              ctypes.c_long, ctypes.POINTER(ctypes.c_long)]
+        lib.cih_oracle_dash_lines.argtypes = [_P, _P, ctypes.c_long, _P, ctypes.c_long,
+                                              ctypes.POINTER(ctypes.c_long)]
+        lib.cih_oracle_lines_to_runs.argtypes = [_P, ctypes.c_float, ctypes.c_float,
+                                                 ctypes.c_int, _P, ctypes.c_long]
         fn = lib.cih_oracle_measure_text
         fn.restype = ctypes.c_float
         fn.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_float, ctypes.c_char_p]
+    fns["seam"] = "cih_ref_" if kind == "reference" else "cih_oracle_"
     fns["lib"] = lib
     _libs[kind] = fns
     return fns
@@ -143,35 +153,35 @@ class Canvas:
         self.fns["read_linear"](self.handle, out.ctypes.data_as(_P))
         return out
 
+    def _seam(self, name):
+        return getattr(self.fns["lib"], self.fns["seam"] + name)
+
     def path_lines(self, stroking):
-        """Reference seam: lines after path_to_lines (+ stroke_lines)."""
-        lib = self.fns["lib"]
+        """Stage seam: lines after path_to_lines (+ stroke_lines)."""
         cap = 1 << 22
         pts = np.zeros((cap, 2), dtype=np.float32)
         sub = np.zeros((cap // 2, 2), dtype=np.int32)
         ns = ctypes.c_long()
-        n = lib.cih_ref_path_lines(self.handle, int(stroking), pts.ctypes.data_as(_P), cap,
-                                   sub.ctypes.data_as(_P), cap // 2, ctypes.byref(ns))
+        n = self._seam("path_lines")(self.handle, int(stroking), pts.ctypes.data_as(_P), cap,
+                                     sub.ctypes.data_as(_P), cap // 2, ctypes.byref(ns))
         return pts[:n].copy(), sub[:ns.value].copy()
 
     def dash_lines(self):
-        """Reference seam: lines after path_to_lines(true) and dash_lines."""
-        lib = self.fns["lib"]
+        """Stage seam: lines after path_to_lines(true) and dash_lines."""
         cap = 1 << 22
         pts = np.zeros((cap, 2), dtype=np.float32)
         sub = np.zeros((cap // 2, 2), dtype=np.int32)
         ns = ctypes.c_long()
-        n = lib.cih_ref_dash_lines(self.handle, pts.ctypes.data_as(_P), cap,
-                                   sub.ctypes.data_as(_P), cap // 2, ctypes.byref(ns))
+        n = self._seam("dash_lines")(self.handle, pts.ctypes.data_as(_P), cap,
+                                     sub.ctypes.data_as(_P), cap // 2, ctypes.byref(ns))
         return pts[:n].copy(), sub[:ns.value].copy()
 
     def lines_to_runs(self, offset_x=0.0, offset_y=0.0, padding=0):
-        """Reference seam: sorted, coalesced runs of the current lines."""
-        lib = self.fns["lib"]
+        """Stage seam: sorted, coalesced runs of the current lines."""
         cap = 1 << 23
         runs = np.zeros((cap, 3), dtype=np.float32)
-        n = lib.cih_ref_lines_to_runs(self.handle, offset_x, offset_y, padding,
-                                      runs.ctypes.data_as(_P), cap)
+        n = self._seam("lines_to_runs")(self.handle, offset_x, offset_y, padding,
+                                        runs.ctypes.data_as(_P), cap)
         return runs[:n].copy()
 
     def close(self):

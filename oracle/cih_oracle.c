@@ -2164,3 +2164,59 @@ malformed:
     free(images);
     return -1;
 }
+
+/* ---- stage seams ---------------------------------------------------------
+ * The intermediate data of this restatement's stages, in the layout of the
+ * reference build's seams (oracle/ref_shim.cpp), so tests can compare the
+ * GPU's polygons and runs with them where the reference build is absent. */
+
+static long copy_lines(const cih_oracle *c, float *points, long capacity, int *subpath_info,
+                       long subpath_capacity, long *subpaths)
+{
+    long n = (long)c->lines.points.n, s = (long)c->lines.subpaths.n, k;
+    for (k = 0; k < n && k < capacity; ++k) {
+        points[2 * k + 0] = c->lines.points.v[k].x;
+        points[2 * k + 1] = c->lines.points.v[k].y;
+    }
+    for (k = 0; k < s && k < subpath_capacity; ++k) {
+        subpath_info[2 * k + 0] = (int)c->lines.subpaths.v[k].count;
+        subpath_info[2 * k + 1] = c->lines.subpaths.v[k].closed ? 1 : 0;
+    }
+    *subpaths = s;
+    return n;
+}
+
+/* path_to_lines (hpp:1495) of the current path and, when stroking,
+ * stroke_lines (hpp:2070), which dashes first. */
+long cih_oracle_path_lines(cih_oracle *c, int stroking, float *points, long capacity,
+                           int *subpath_info, long subpath_capacity, long *subpaths)
+{
+    path_to_lines(c, stroking != 0);
+    if (stroking)
+        stroke_lines(c);
+    return copy_lines(c, points, capacity, subpath_info, subpath_capacity, subpaths);
+}
+
+/* path_to_lines(true) (hpp:1495) and dash_lines (hpp:1858), unstroked. */
+long cih_oracle_dash_lines(cih_oracle *c, float *points, long capacity, int *subpath_info,
+                           long subpath_capacity, long *subpaths)
+{
+    path_to_lines(c, 1);
+    dash_lines(c);
+    return copy_lines(c, points, capacity, subpath_info, subpath_capacity, subpaths);
+}
+
+/* lines_to_runs (hpp:2193) of the current lines: (x, y, delta) triples. */
+long cih_oracle_lines_to_runs(cih_oracle *c, float offset_x, float offset_y, int padding,
+                              float *runs, long capacity)
+{
+    long n, k;
+    lines_to_runs(c, mkxy(offset_x, offset_y), padding);
+    n = (long)c->runs.n;
+    for (k = 0; k < n && k < capacity; ++k) {
+        runs[3 * k + 0] = (float)c->runs.v[k].x;
+        runs[3 * k + 1] = (float)c->runs.v[k].y;
+        runs[3 * k + 2] = c->runs.v[k].delta;
+    }
+    return n;
+}

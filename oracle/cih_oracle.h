@@ -34,6 +34,17 @@ float cih_oracle_measure_text(const unsigned char *font, size_t bytes, float siz
                               const char *text);
 /* The fp32 linear premultiplied RGBA bitmap. */
 void cih_oracle_read_linear(cih_oracle *canvas, float *rgba);
+/* Stage seams, laid out as the reference build's (oracle/ref_shim.cpp):
+ * lines after path_to_lines (and stroke_lines), lines after dash_lines, and
+ * the sorted, merged runs of the current lines.  Up to `capacity` points
+ * (x, y) and `subpath_capacity` (count, closed) pairs are written; the
+ * return value counts all points (runs), *subpaths all subpaths. */
+long cih_oracle_path_lines(cih_oracle *canvas, int stroking, float *points, long capacity,
+                           int *subpath_info, long subpath_capacity, long *subpaths);
+long cih_oracle_dash_lines(cih_oracle *canvas, float *points, long capacity, int *subpath_info,
+                           long subpath_capacity, long *subpaths);
+long cih_oracle_lines_to_runs(cih_oracle *canvas, float offset_x, float offset_y, int padding,
+                              float *runs, long capacity);
 
 #ifdef __cplusplus
 }

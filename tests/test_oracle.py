@@ -62,6 +62,84 @@ def test_oracle_bit_identical_to_reference(cfg, seed, w, h, n, flags):
     assert np.array_equal(ref8, got8)
 
 
+def seam_stream(seed):
+    """A seeded five-segment bezier path, closed on two seeds of three, with
+    a seeded line width, join, cap and (odd seeds) dash pattern."""
+    from canvas_ity_amd.stream import Stream
+    rng = np.random.default_rng(seed)
+    s = Stream(160, 160)
+    s.set_line_width(float(rng.uniform(1, 12))).set_line_join(seed % 3).set_line_cap(seed % 3)
+    if seed % 2:
+        s.set_line_dash([float(v) for v in rng.uniform(2, 30, 3)]).set_line_dash_offset(3.3)
+    s.begin_path().move_to(*rng.uniform(-10, 170, 2))
+    for _ in range(5):
+        s.bezier_curve_to(*rng.uniform(-10, 170, 6))
+    if seed % 3:
+        s.close_path()
+    return s.tobytes()
+
+
+def seams(stream, kind):
+    """(name, array) of every stage seam of a stream's path."""
+    out = []
+    for stroking in (False, True):
+        c = oracle.Canvas(160, 160, kind)
+        c.replay(stream)
+        pts, subs = c.path_lines(stroking)
+        out += [("lines", pts), ("subpaths", subs), ("runs", c.lines_to_runs()),
+                ("runs", c.lines_to_runs(0.5, -2.0, 1))]
+    c = oracle.Canvas(160, 160, kind)
+    c.replay(stream)
+    pts, subs = c.dash_lines()
+    return out + [("lines", pts), ("subpaths", subs)]
+
+
+@pytest.mark.skipif(not REFERENCE, reason="reference build (oracle/_ref) not present")
+@pytest.mark.parametrize("seed", range(12))
+def test_stage_seams_match_the_reference(seed):
+    # the oracle's polygons (filled, stroked, dashed) bit for bit, and its
+    # runs: the same pixels in the same order, and deltas within 1e-6.  The
+    # reference merges runs after std::sort, which leaves the order of runs it
+    # ranks equal (same pixel and |delta|) unspecified, so a merged delta may
+    # differ in its last bits; GPU tests compare with these seams where the
+    # reference build is absent
+    stream = seam_stream(seed)
+    for k, ((name, ref), (_, got)) in enumerate(zip(seams(stream, "reference"),
+                                                    seams(stream, "port"))):
+        assert ref.shape == got.shape and ref.dtype == got.dtype, (k, name)
+        if name != "runs":
+            assert np.array_equal(ref.view(np.uint32), got.view(np.uint32)), (k, name)
+            continue
+        assert np.array_equal(ref[:, :2], got[:, :2]), k
+        assert np.abs(ref[:, 2] - got[:, 2]).max(initial=0.0) <= 1e-6, k
+        assert np.count_nonzero(ref[:, 2] != got[:, 2]) <= max(2, len(ref) // 200), k
+
+
+@pytest.mark.parametrize("x0", [4.0, 4.5, 4.25])
+def test_stage_seams_closed_form(x0):
+    # a rectangle from (x0, 5) to (6, 8): one closed subpath of its corners,
+    # and runs whose running sum along each covered row is minus the covered
+    # fraction of each pixel (the path is clockwise on screen), 0 from x = 6
+    from canvas_ity_amd.stream import Stream
+    s = Stream(16, 16)
+    s.begin_path().move_to(x0, 5).line_to(6, 5).line_to(6, 8).line_to(x0, 8).close_path()
+    c = oracle.Canvas(16, 16, "port")
+    c.replay(s.tobytes())
+    pts, subs = c.path_lines(False)
+    assert subs[0].tolist() == [5, 1]
+    assert pts[:5].tolist() == [[x0, 5], [6, 5], [6, 8], [x0, 8], [x0, 5]]
+    runs = c.lines_to_runs()
+    assert set(runs[:, 1].astype(int).tolist()) == {5, 6, 7}
+    for y in (5, 6, 7):
+        row = runs[runs[:, 1] == y]
+        sums = np.zeros(16)
+        for x, _, d in row:
+            sums[int(x):] += d
+        expect = np.clip(np.minimum(np.arange(16) + 1.0, 6.0) - np.maximum(np.arange(16), x0),
+                         0.0, 1.0)
+        assert np.array_equal(-sums, expect), (y, sums.tolist())
+
+
 def test_fractional_rectangle_coverage():
     rect = (3.25, 5.6, 17.3, 9.15)
     _, lin = render(scenes.fractional_rect(32, rect))
