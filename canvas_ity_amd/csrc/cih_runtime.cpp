@@ -193,7 +193,11 @@ struct exec_lane {
     dbuf points, words, runs, steps, shadow, counters; // arenas and their counters
     dbuf state;                                  // per-draw results and the work list
     hbuf host_counters;
-    size_t last_ndraws = 0; // draws of the lane's last frame (for the trace seams)
+    // the lane's last frame, for the trace seams: its draw count, draw
+    // records and packet payload (device memory)
+    size_t last_ndraws = 0;
+    const draw_desc *last_draws = nullptr;
+    const unsigned char *last_payload = nullptr;
     void release() {
         dbuf *all[] = {&points, &words, &runs, &steps, &shadow, &counters, &state};
         for (dbuf *b : all)
@@ -547,6 +551,8 @@ int run_kernels(cih_device *d, exec_lane &L, const unsigned char *payload,
             return s;
     }
     L.last_ndraws = n;
+    L.last_draws = draws;
+    L.last_payload = payload;
     for (int attempt = 0; attempt < 8; ++attempt) {
         frame F;
         std::memset(&F, 0, sizeof(F));
@@ -1245,13 +1251,33 @@ int cih_trace_stage(cih_canvas *c, int draw, int stage, float *points, size_t po
     std::lock_guard<std::recursive_mutex> lock(d->mutex);
     if ((size_t)draw >= d->main.last_ndraws)
         return fail(CIH_ERROR_INVALID, "no such draw in the last frame");
+    if (stage < 0 || stage > 2)
+        return fail(CIH_ERROR_INVALID, "stage must be 0 (flatten), 1 (dash) or 2 (polygons)");
     draw_state S;
+    draw_desc D;
     HIP_TRY(copy_sync(d, &S, (draw_state *)d->main.state.ptr + draw, sizeof(S),
                       hipMemcpyDeviceToHost));
-    uint64_t pts = stage == 0 ? S.flat_points : stage == 1 ? S.dash_points : S.poly_points;
-    uint64_t cnt = stage == 0 ? S.flat_counts : stage == 1 ? S.dash_counts : S.poly_counts;
-    size_t np = stage == 0 ? S.flat_npoints : stage == 1 ? S.dash_npoints : S.poly_npoints;
-    size_t ns = stage == 0 ? S.flat_nsub : stage == 1 ? S.dash_nsub : S.poly_nsub;
+    HIP_TRY(copy_sync(d, &D, d->main.last_draws + draw, sizeof(D), hipMemcpyDeviceToHost));
+    // k_geometry records a stage only where it runs: flattening for paths and
+    // clips (a polyline draw's own lines are its flatten stage), dashing for
+    // dashed strokes; the other stages of a draw are empty
+    bool path = D.kind == CIH_DRAW_FILL_PATH || D.kind == CIH_DRAW_STROKE_PATH ||
+                D.kind == CIH_DRAW_CLIP;
+    bool lines = D.kind == CIH_DRAW_FILL_LINES || D.kind == CIH_DRAW_STROKE_LINES;
+    bool stroke = D.kind == CIH_DRAW_STROKE_PATH || D.kind == CIH_DRAW_STROKE_LINES;
+    uint64_t pts = 0, cnt = 0;
+    size_t np = 0, ns = 0;
+    if (stage == 0 && path) {
+        pts = S.flat_points, cnt = S.flat_counts, np = S.flat_npoints, ns = S.flat_nsub;
+    } else if (stage == 0 && lines) {
+        pts = (uint64_t)(uintptr_t)(d->main.last_payload + D.points_at);
+        cnt = (uint64_t)(uintptr_t)(d->main.last_payload + D.subpaths_at);
+        np = D.npoints, ns = D.nsubpaths;
+    } else if (stage == 1 && stroke && D.ndash) {
+        pts = S.dash_points, cnt = S.dash_counts, np = S.dash_npoints, ns = S.dash_nsub;
+    } else if (stage == 2 && (path || lines)) {
+        pts = S.poly_points, cnt = S.poly_counts, np = S.poly_npoints, ns = S.poly_nsub;
+    }
     *npoints = np;
     *nsubpaths = ns;
     if (points && point_capacity && np)

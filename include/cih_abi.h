@@ -196,9 +196,10 @@ CIH_API int cih_device_profile_read(cih_device *device, char *json, size_t capac
 CIH_API int cih_trace_polygons(cih_canvas *canvas, int draw, float *points,
                                size_t point_capacity, uint32_t *subpaths,
                                size_t subpath_capacity, size_t *npoints, size_t *nsubpaths);
-/* cih_trace_stage: stage 0 flattening output (path_to_lines, hpp:1495), 1
- * dashing output (dash_lines, hpp:1858; strokes with a dash pattern only),
- * 2 the final polygons (as cih_trace_polygons). */
+/* cih_trace_stage: stage 0 flattening output (path_to_lines, hpp:1495; a
+ * polyline draw's own lines), 1 dashing output (dash_lines, hpp:1858; empty
+ * unless the draw is a stroke with a dash pattern), 2 the final polygons (as
+ * cih_trace_polygons).  A stage a draw does not have is empty. */
 CIH_API int cih_trace_stage(cih_canvas *canvas, int draw, int stage, float *points,
                             size_t point_capacity, uint32_t *subpaths, size_t subpath_capacity,
                             size_t *npoints, size_t *nsubpaths);

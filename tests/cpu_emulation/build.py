@@ -202,14 +202,36 @@ TIGHT_RERUN = """        if (!hc[COUNTER_OVERFLOW]) {
                 attempt = 6;
                 continue;
             }
-            // arenas back to at least their earlier sizes: later unchecked
-            // runs of other frames rely on arenas that only grow
-            if (regrow && (s = ensure_arenas(L, keep[0], keep[1], keep[2], keep[3])))
-                return s;
+            // arenas back to at least their earlier sizes before the lane's
+            // next frame (later unchecked runs rely on arenas that only
+            // grow); until then the trace seams read this frame's arenas
+            if (regrow) {
+                std::lock_guard<std::mutex> lock(emu_regrow_mutex);
+                emu_regrow[&L] = {keep[0], keep[1], keep[2], keep[3]};
+            }
             return CIH_OK;
         }"""
 
 SAVE_CANVASES = """    L.last_ndraws = n;
+    // emulation: arenas that a tight re-run of the lane's previous frame left
+    // small grow back first
+    static std::mutex emu_regrow_mutex;
+    static std::map<const exec_lane *, std::array<uint64_t, 4>> emu_regrow;
+    {
+        std::array<uint64_t, 4> grow = {0, 0, 0, 0};
+        bool pending = false;
+        {
+            std::lock_guard<std::mutex> lock(emu_regrow_mutex);
+            auto it = emu_regrow.find(&L);
+            if (it != emu_regrow.end()) {
+                grow = it->second;
+                pending = true;
+                emu_regrow.erase(it);
+            }
+        }
+        if (pending && (s = ensure_arenas(L, grow[0], grow[1], grow[2], grow[3])))
+            return s;
+    }
     uint64_t keep[4] = {0, 0, 0, 0};
     bool regrow = false;
     std::vector<std::vector<unsigned char>> saved(ncanvases);
@@ -242,6 +264,8 @@ UNPOISON_ARENAS = """        // emulation: arena red zones of an earlier frame a
 def emulated_runtime(src):
     """cih_runtime.cpp with exact-size buffers and a tight re-run of each
     checked frame."""
+    src = "#include <array>\n" + src
+
     def sub(old, new):
         nonlocal src
         assert old in src, old[:60]

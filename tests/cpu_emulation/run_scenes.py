@@ -323,7 +323,29 @@ def serve_burst(device):
             assert np.array_equal(alone[id(s)], got[rid]), (budget, max_batch, rid)
 
 
-MODES = {"batch_and_server": batch_and_server, "batch_small": batch_small, "serve_burst": serve_burst,
+def round_joins(device):
+    """Strokes with round joins (scenes.round_join_strokes): the kernels'
+    polygons equal the oracle's stroked lines bit for bit, and their fp32
+    pixels the oracle's render (the join's acos and tan as glibc computes
+    them, which the oracle and the reference call)."""
+    for name, (one, before) in scenes.round_join_strokes().items():
+        w, h = cia.stream_size(one)
+        ref = oracle.Canvas(w, h, "port")
+        ref.replay(before)
+        rp, rs = ref.path_lines(True)
+        c = cia.Canvas(w, h, device)
+        c.replay(one)
+        c.flush()
+        gp, gs = c.trace_polygons(0)
+        c.close()
+        assert np.array_equal(rs, gs.astype(np.int32)), name
+        assert np.array_equal(rp.view(np.uint32), gp.view(np.uint32)), name
+        _, pl = oracle.render(one, "port")
+        _, gl = cia.render(one, device)
+        assert np.array_equal(pl.view(np.uint32), gl.view(np.uint32)), name
+
+
+MODES = {"batch_and_server": batch_and_server, "round_joins": round_joins, "batch_small": batch_small, "serve_burst": serve_burst,
          "batch_frames": batch_frames, "batch_frames_render": batch_frames_render,
          "payload_limit": payload_limit,
          "fine_configs": fine_configs, "canvas_execute": canvas_execute,

@@ -852,3 +852,49 @@ def put_scene(seed, size=48):
     s.set_composite_op(COMPOSITE["source_copy"])
     s.fill_rectangle(20.0, 20.0, 10.0, 10.0)
     return s.tobytes()
+
+
+def single_draw(stream, k):
+    """Draw k (1-based) of a command stream on its own: (the stream with every
+    state and path record before that draw and the draw itself, the same
+    without the draw).  Paths start with begin_path, so dropping the earlier
+    draws leaves the state and path draw k sees unchanged."""
+    from canvas_ity_amd.stream import DRAW_OPS, records
+    words = np.frombuffer(stream, dtype=np.uint32)
+    keep = [words[:4]]
+    seen = 0
+    for code, at, n in records(stream):
+        if code in DRAW_OPS:
+            seen += 1
+            if seen == k:
+                before = np.concatenate(keep).tobytes()
+                return np.concatenate(keep + [words[at:at + 1 + n]]).tobytes(), before
+            continue
+        keep.append(words[at:at + 1 + n])
+    raise IndexError("the stream has %d draws" % seen)
+
+
+def round_join_strokes():
+    """Strokes whose round joins go through acosf/tanf (canvas_ity.hpp:
+    1994-1997): draw 157 of batch-mix canvas 40 (seed 0xC4), whose join a
+    correctly rounded acos moves by a few ulps, and seeded five-segment paths
+    with round joins and each cap, open and closed."""
+    import canvas_ity_amd as cia
+    out = {"mix40_157": single_draw(cia.scene(4, cia.scene_seed(0xC4, 40)), 157)}
+    for seed in range(6):
+        rng = np.random.default_rng(0x501 + seed)
+        s = Stream(96, 80)
+        s.set_line_width(float(rng.uniform(2, 14))).set_line_join(2).set_line_cap(seed % 3)
+        s.begin_path().move_to(*rng.uniform(-5, 100, 2))
+        for _ in range(5):
+            if seed % 2:
+                s.line_to(*rng.uniform(-5, 100, 2))
+            else:
+                s.bezier_curve_to(*rng.uniform(-5, 100, 6))
+        if seed % 3 == 1:
+            s.close_path()
+        before = s.tobytes()
+        s.set_color(1, 0.2, 0.5, 0.9, 0.8)  # stroke_style
+        s.stroke()
+        out["seeded_%d" % seed] = (s.tobytes(), before)
+    return out

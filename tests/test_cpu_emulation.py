@@ -114,6 +114,9 @@ RUNS = [
     # over many canvases in one frame) is cut into frames within the
     # multi-canvas bounds; above 2^30 pixels a batch is refused
     ("frames", dict(names=["batch_frames"])),
+    # strokes with round joins: polygons and fp32 pixels bit for bit equal
+    # to the oracle's (the join's acos and tan evaluated as glibc does)
+    ("round_joins", dict(names=["round_joins"], threads=4)),
     # puts recorded as packets in batch and server canvases, applied in
     # painter's order: the same pixels as the plain canvas, which applies them
     # at once
@@ -183,3 +186,7 @@ def test_arena_growth_after_overflow(runs):
 
 def test_deferred_streams_and_a_serving_burst(runs):
     runs["deferred"].result()
+
+
+def test_round_joins_bit_for_bit(runs):
+    runs["round_joins"].result()
