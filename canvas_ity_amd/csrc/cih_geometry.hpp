@@ -54,7 +54,7 @@ CIH_HD f2 lerp(f2 from, f2 to, float ratio) { return add(from, mul(ratio, sub(to
 //   Developed at SunPro, a Sun Microsystems, Inc. business.
 //   Permission to use, copy, modify, and distribute this software is freely
 //   granted, provided that this notice is preserved.
-// tests/test_join_math.py compares them with the host libm's acosf over all
+// tests/test_libm_parity.py compares them with the host libm's acosf over all
 // of [-1, 1] and tanf over all of [0, pi/4] (in strides, or every input).
 CIH_HD uint32_t float_bits(float v) {
     uint32_t u;

@@ -31,6 +31,7 @@
 
 #include "cih_device.hpp"
 #include "cih_geometry.hpp"
+#include "cih_color.hpp"
 #include "cih_blur.hpp"
 #include "cih_abi.h"
 
@@ -2438,13 +2439,6 @@ __global__ void k_put_image(const uint32_t *rgba8, int w, int h, float4 *bitmap,
             continue;
         bitmap[(size_t)cy * W + cx] = texel_from_rgba8(rgba8[i], lut);
     }
-}
-
-// sRGB encode (hpp:1282-1284).  The power is evaluated in double and rounded
-// once (correctly rounded), as close as a device can get to the host's powf.
-__device__ inline float delinearized(float v) {
-    return v < 0.0031308f ? 12.92f * v
-                          : 1.055f * (float)pow((double)v, (double)(1.0f / 2.4f)) - 0.055f;
 }
 
 __device__ inline float clamp01(float v) { return smin(smax(v, 0.0f), 1.0f); }

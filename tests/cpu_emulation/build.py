@@ -324,7 +324,8 @@ def emulated_runtime(src):
 
 def _sources():
     names = ["cih_kernels.hip", "cih_runtime.cpp", "cih_replay.cpp", "cih_scene.cpp",
-             "cih_text.cpp", "cih_device.hpp", "cih_geometry.hpp", "cih_blur.hpp"]
+             "cih_text.cpp", "cih_device.hpp", "cih_geometry.hpp", "cih_blur.hpp",
+             "cih_color.hpp"]
     h = hashlib.sha256()
     for n in names:
         h.update(open(os.path.join(CSRC, n), "rb").read())
@@ -374,8 +375,11 @@ def build(force=False):
         out, _ = p.communicate()
         if p.returncode:
             raise RuntimeError("emulation build failed:\n" + out.decode()[-4000:])
-    subprocess.run(["g++", "-shared", "-fsanitize=address", "-o", LIBRARY] + objects +
+    # linked beside and renamed into place: a process that has the old
+    # library loaded keeps it
+    subprocess.run(["g++", "-shared", "-fsanitize=address", "-o", LIBRARY + ".tmp"] + objects +
                    ["-lpthread"], check=True)
+    os.replace(LIBRARY + ".tmp", LIBRARY)
     with open(stamp, "w") as f:
         f.write(digest)
     return LIBRARY

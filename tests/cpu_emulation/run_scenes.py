@@ -345,7 +345,33 @@ def round_joins(device):
         assert np.array_equal(pl.view(np.uint32), gl.view(np.uint32)), name
 
 
-MODES = {"batch_and_server": batch_and_server, "round_joins": round_joins, "batch_small": batch_small, "serve_burst": serve_burst,
+def trace_stages(device):
+    """cih_trace_stage reports only the stages a draw has: an undashed
+    stroke's dash stage is empty, a dashed one's is not, and a rectangle
+    (a polyline draw) has its own corners as its flatten stage."""
+    from canvas_ity_amd.stream import Stream
+    s = Stream(40, 30)
+    s.fill_rectangle(3.5, 4.0, 20.0, 10.0)
+    s.begin_path().move_to(2, 20).line_to(30, 25).line_to(35, 5)
+    s.stroke()
+    s.set_line_dash([4.0, 3.0])
+    s.stroke()
+    c = cia.Canvas(40, 30, device)
+    c.replay(s.tobytes())
+    c.flush()
+    pts, subs = c.trace_stage(0, 0)
+    assert subs.shape[0] == 1 and int(subs[0, 0]) == len(pts), (pts, subs)
+    corners = {(3.5, 4.0), (23.5, 4.0), (23.5, 14.0), (3.5, 14.0)}
+    assert {tuple(map(float, p)) for p in pts} == corners, pts
+    assert c.trace_stage(0, 1)[0].shape[0] == 0
+    assert c.trace_stage(1, 1)[0].shape[0] == 0 and c.trace_stage(1, 1)[1].shape[0] == 0
+    dashed, dsubs = c.trace_stage(2, 1)
+    assert dashed.shape[0] > 0 and dsubs.shape[0] > 1, (dashed.shape, dsubs.shape)
+    c.close()
+
+
+MODES = {"batch_and_server": batch_and_server, "round_joins": round_joins,
+         "trace_stages": trace_stages, "batch_small": batch_small, "serve_burst": serve_burst,
          "batch_frames": batch_frames, "batch_frames_render": batch_frames_render,
          "payload_limit": payload_limit,
          "fine_configs": fine_configs, "canvas_execute": canvas_execute,

@@ -116,7 +116,7 @@ RUNS = [
     ("frames", dict(names=["batch_frames"])),
     # strokes with round joins: polygons and fp32 pixels bit for bit equal
     # to the oracle's (the join's acos and tan evaluated as glibc does)
-    ("round_joins", dict(names=["round_joins"], threads=4)),
+    ("round_joins", dict(names=["round_joins", "trace_stages"], threads=4)),
     # puts recorded as packets in batch and server canvases, applied in
     # painter's order: the same pixels as the plain canvas, which applies them
     # at once

@@ -1,13 +1,19 @@
-"""The round-join acos and tan of the geometry kernels against the host libm.
+"""The kernels' libm-dependent arithmetic against the host libm.
 
 The reference computes a round join's cubic from acosf and tanf
 (canvas_ity.hpp:1994-1997), so the join's points depend on their last bits.
 canvas_ity_amd/csrc/cih_geometry.hpp evaluates both as glibc does (the float
 versions of fdlibm's e_acos.c and k_tan.c), the same code on host and device.
-This compiles that header on the host with the library's floating-point flags
-and compares join_acos with acosf on [-1, 1] and join_tan with tanf on
-[0, float(pi)/4], bit for bit: every 101st input plus every branch edge, or
-every input with CIH_EXHAUSTIVE_JOIN_MATH=1 (about a minute).
+Read-back encodes sRGB with powf(v, 1/2.4) (hpp:1282-1284);
+canvas_ity_amd/csrc/cih_color.hpp evaluates it in double, rounded once, except
+on the four inputs where glibc's rounding changes a dithered byte.
+
+This compiles both headers on the host with the library's floating-point
+flags and compares join_acos with acosf on [-1, 1] and join_tan with tanf on
+[0, float(pi)/4] bit for bit, and the read-back byte of every dither offset
+with the reference's expression on [0.0031308, 1]: every 101st input plus
+every branch edge and the known special inputs, or every input with
+CIH_EXHAUSTIVE_JOIN_MATH=1 (about a minute).
 """
 import os
 import shutil
@@ -19,7 +25,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "canvas_ity_amd", "csrc")
 
 PROGRAM = r"""
-#include "cih_geometry.hpp"
+#include "cih_color.hpp"
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -49,8 +55,30 @@ static void tan_at(uint32_t u) {
     }
 }
 
+// the reference's read-back byte (hpp:1282-1284, 3373-3379) against the
+// kernels' for each of the 16 ordered-dither offsets (hpp:3358-3362)
+static void srgb_at(uint32_t u) {
+    float x = bits_float(u);
+    float want = 1.055f * powf(x, 1.0f / 2.4f) - 0.055f, got = cih::delinearized(x);
+    ++checked;
+    for (int k = 0; k < 16; ++k) {
+        float t = ((float)k + 0.5f) / 16.0f;
+        if ((int)(t + want * 255.0f) != (int)(t + got * 255.0f)) {
+            if (wrong++ < 5)
+                printf("srgb(%a) dither %d: libm %a, kernels %a\n", x, k, want, got);
+            return;
+        }
+    }
+}
+
 int main(int argc, char **argv) {
     uint32_t stride = (uint32_t)strtoul(argv[1], 0, 0);
+    for (uint64_t u = float_bits(0.0031308f); u <= 0x3f800000u; u += stride)
+        srgb_at((uint32_t)u);
+    const uint32_t srgb_special[] = {0x3c8979b7u, 0x3e17c58du, 0x3e48b85au, 0x3e7ec3d6u,
+                                     0x3c8979b6u, 0x3c8979b8u};
+    for (uint32_t u : srgb_special)
+        srgb_at(u);
     for (uint64_t u = 0; u <= 0x3f800000u; u += stride) {
         acos_at((uint32_t)u);
         acos_at((uint32_t)u | 0x80000000u);
@@ -81,7 +109,7 @@ int main(int argc, char **argv) {
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="g++ is not available")
-def test_join_acos_and_tan_match_the_host_libm(tmp_path):
+def test_join_math_and_srgb_encoding_match_the_host_libm(tmp_path):
     src = tmp_path / "join_math.cpp"
     src.write_text(PROGRAM)
     exe = tmp_path / "join_math"
