@@ -338,52 +338,79 @@ def _sources():
 
 
 def asan_runtime():
-    return subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True,
+    return sanitizer_runtime("address")
+
+
+def sanitizer_runtime(sanitizer):
+    """The sanitizer's runtime library, to preload into the Python process."""
+    name = {"address": "libasan.so", "thread": "libtsan.so"}[sanitizer]
+    return subprocess.run(["g++", "-print-file-name=" + name], capture_output=True,
                           text=True, check=True).stdout.strip()
 
 
-def build(force=False):
-    """Builds OUT/libcanvas_ity_hip.so unless it is current; returns its path."""
-    os.makedirs(OUT, exist_ok=True)
-    stamp = os.path.join(OUT, "sources.sha256")
+def build(force=False, sanitizer="address"):
+    """Builds the emulation library unless it is current; returns its path.
+
+    sanitizer="address" (OUT/libcanvas_ity_hip.so) instruments everything
+    with AddressSanitizer.  sanitizer="thread" (OUT_thread/...) instruments
+    the host runtime, replay, scene and text code with ThreadSanitizer and
+    leaves the emulated kernels and the emulated HIP runtime uninstrumented
+    (their lanes switch stacks, which ThreadSanitizer does not follow): it
+    checks the host code's threads (server workers and scheduler, batch
+    replay threads, callers) for data races, not the kernels."""
+    out = OUT if sanitizer == "address" else OUT + "_" + sanitizer
+    library = os.path.join(out, "libcanvas_ity_hip.so")
+    os.makedirs(out, exist_ok=True)
+    stamp = os.path.join(out, "sources.sha256")
     digest = _sources()
-    if not force and os.path.exists(LIBRARY) and os.path.exists(stamp) and \
+    if not force and os.path.exists(library) and os.path.exists(stamp) and \
             open(stamp).read() == digest:
-        return LIBRARY
-    with open(os.path.join(OUT, "emu_kernels.cpp"), "w") as f:
+        return library
+    with open(os.path.join(out, "emu_kernels.cpp"), "w") as f:
         f.write(emulated_kernels(open(os.path.join(CSRC, "cih_kernels.hip")).read()))
-    with open(os.path.join(OUT, "emu_runtime.cpp"), "w") as f:
-        f.write(emulated_runtime(open(os.path.join(CSRC, "cih_runtime.cpp")).read()))
-    flags = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-fsanitize=address",
-             "-fno-omit-frame-pointer", "-ffp-contract=off", "-I" + HERE,
-             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+    runtime = emulated_runtime(open(os.path.join(CSRC, "cih_runtime.cpp")).read())
+    if sanitizer == "thread":
+        # timed condition-variable waits through pthread_cond_timedwait, which
+        # this ThreadSanitizer intercepts (it misses pthread_cond_clockwait and
+        # would report every mutex held across such a wait)
+        runtime = ("#include <bits/c++config.h>\n#undef _GLIBCXX_USE_PTHREAD_COND_CLOCKWAIT\n" +
+                   runtime)
+    with open(os.path.join(out, "emu_runtime.cpp"), "w") as f:
+        f.write(runtime)
+    flags = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-fno-omit-frame-pointer",
+             "-ffp-contract=off", "-I" + HERE, "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+    host = ["-fsanitize=" + sanitizer]
+    emulated = host if sanitizer == "address" else []
     jobs = [
-        (os.path.join(OUT, "emu_kernels.cpp"), []),
-        (os.path.join(OUT, "emu_runtime.cpp"), ["-fpermissive", "-fvisibility=hidden", "-w"]),
-        (os.path.join(CSRC, "cih_replay.cpp"), ["-fvisibility=hidden"]),
-        (os.path.join(CSRC, "cih_scene.cpp"), ["-fvisibility=hidden"]),
-        (os.path.join(CSRC, "cih_text.cpp"), ["-fvisibility=hidden"]),
-        (os.path.join(HERE, "emu_rt.cpp"), ["-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0"]),
+        (os.path.join(out, "emu_kernels.cpp"), emulated),
+        (os.path.join(out, "emu_runtime.cpp"), host + ["-fpermissive", "-fvisibility=hidden",
+                                                      "-w"]),
+        (os.path.join(CSRC, "cih_replay.cpp"), host + ["-fvisibility=hidden"]),
+        (os.path.join(CSRC, "cih_scene.cpp"), host + ["-fvisibility=hidden"]),
+        (os.path.join(CSRC, "cih_text.cpp"), host + ["-fvisibility=hidden"]),
+        (os.path.join(HERE, "emu_rt.cpp"), emulated + ["-U_FORTIFY_SOURCE",
+                                                       "-D_FORTIFY_SOURCE=0"]),
     ]
     procs, objects = [], []
     for src, extra in jobs:
-        obj = os.path.join(OUT, os.path.basename(src).split(".")[0] + ".o")
+        obj = os.path.join(out, os.path.basename(src).split(".")[0] + ".o")
         objects.append(obj)
         procs.append(subprocess.Popen(flags + extra + ["-c", src, "-o", obj],
                                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT))
     for p in procs:
-        out, _ = p.communicate()
+        text, _ = p.communicate()
         if p.returncode:
-            raise RuntimeError("emulation build failed:\n" + out.decode()[-4000:])
+            raise RuntimeError("emulation build failed:\n" + text.decode()[-4000:])
     # linked beside and renamed into place: a process that has the old
     # library loaded keeps it
-    subprocess.run(["g++", "-shared", "-fsanitize=address", "-o", LIBRARY + ".tmp"] + objects +
-                   ["-lpthread"], check=True)
-    os.replace(LIBRARY + ".tmp", LIBRARY)
+    subprocess.run(["g++", "-shared", "-fsanitize=" + sanitizer, "-o", library + ".tmp"] +
+                   objects + ["-lpthread"], check=True)
+    os.replace(library + ".tmp", library)
     with open(stamp, "w") as f:
         f.write(digest)
-    return LIBRARY
+    return library
 
 
 if __name__ == "__main__":
-    print(build(force="--force" in sys.argv))
+    print(build(force="--force" in sys.argv,
+                sanitizer="thread" if "--thread" in sys.argv else "address"))

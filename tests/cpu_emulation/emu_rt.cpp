@@ -21,6 +21,17 @@
 #include <sanitizer/asan_interface.h>
 #include <sanitizer/common_interface_defs.h>
 
+// AddressSanitizer is told about every switch between lane stacks; a build
+// without it (the ThreadSanitizer build, which instruments the host runtime
+// only) switches without the annotations
+#if defined(__SANITIZE_ADDRESS__)
+#define EMU_FIBER_START(fake, bottom, size) __sanitizer_start_switch_fiber(fake, bottom, size)
+#define EMU_FIBER_FINISH(fake, bottom, size) __sanitizer_finish_switch_fiber(fake, bottom, size)
+#else
+#define EMU_FIBER_START(fake, bottom, size) ((void)(fake), (void)(bottom), (void)(size))
+#define EMU_FIBER_FINISH(fake, bottom, size) ((void)(fake), (void)(bottom), (void)(size))
+#endif
+
 namespace emu {
 
 // A lane is a fiber that lives for the whole process: started once (the only
@@ -121,10 +132,10 @@ void fail(const char *msg) {
 static void to_scheduler() {
     fiber *f = cur;
     if (!__builtin_setjmp(f->jb)) {
-        __sanitizer_start_switch_fiber(&f->fake, sched_bottom, sched_size);
+        EMU_FIBER_START(&f->fake, sched_bottom, sched_size);
         jump_to(sched_jb);
     }
-    __sanitizer_finish_switch_fiber(f->fake, nullptr, nullptr);
+    EMU_FIBER_FINISH(f->fake, nullptr, nullptr);
 }
 
 static void complete(rendezvous &r) {
@@ -178,7 +189,7 @@ void block_barrier(int site) {
 }
 
 static void fiber_entry() {
-    __sanitizer_finish_switch_fiber(nullptr, &sched_bottom, &sched_size);
+    EMU_FIBER_FINISH(nullptr, &sched_bottom, &sched_size);
     for (;;) {
         fiber *f = cur;
         g_body(g_ctx);
@@ -193,10 +204,10 @@ static void fiber_entry() {
         if (block.arrived && block.arrived == block.live)
             complete(block);
         if (!__builtin_setjmp(f->idle)) { // park until the next workgroup
-            __sanitizer_start_switch_fiber(&f->fake, sched_bottom, sched_size);
+            EMU_FIBER_START(&f->fake, sched_bottom, sched_size);
             jump_to(sched_jb);
         }
-        __sanitizer_finish_switch_fiber(cur->fake, &sched_bottom, &sched_size);
+        EMU_FIBER_FINISH(cur->fake, &sched_bottom, &sched_size);
     }
 }
 
@@ -239,7 +250,7 @@ static void run_block(unsigned bx, unsigned by, unsigned bz) {
             if (!__builtin_setjmp(sched_jb)) {
                 if (!f.started && f.alive) { // restart a parked lane
                     f.started = true;
-                    __sanitizer_start_switch_fiber(&sched_fake, f.stack, kStack);
+                    EMU_FIBER_START(&sched_fake, f.stack, kStack);
                     jump_to(f.idle);
                 } else if (!f.started) {
                     f.started = true;
@@ -249,15 +260,15 @@ static void run_block(unsigned bx, unsigned by, unsigned bz) {
                     f.uctx.uc_stack.ss_size = kStack;
                     f.uctx.uc_link = nullptr;
                     makecontext(&f.uctx, fiber_entry, 0);
-                    __sanitizer_start_switch_fiber(&sched_fake, f.stack, kStack);
+                    EMU_FIBER_START(&sched_fake, f.stack, kStack);
                     ucontext_t dummy;
                     swapcontext(&dummy, &f.uctx);
                 } else {
-                    __sanitizer_start_switch_fiber(&sched_fake, f.stack, kStack);
+                    EMU_FIBER_START(&sched_fake, f.stack, kStack);
                     jump_to(f.jb);
                 }
             }
-            __sanitizer_finish_switch_fiber(sched_fake, nullptr, nullptr);
+            EMU_FIBER_FINISH(sched_fake, nullptr, nullptr);
             cur = nullptr;
             if (f.done)
                 remaining--;

@@ -38,10 +38,12 @@ def library():
 STRICT = {"EMU_POISON": "1", "EMU_REDZONES": "1", "EMU_STRICT_STREAMS": "1"}
 
 
-def run(library, names, shuffle=None, strict=False, threads=None, extra=None):
+def run(library, names, shuffle=None, strict=False, threads=None, extra=None,
+        sanitizer="address"):
     env = dict(os.environ)
-    env["LD_PRELOAD"] = emulation.asan_runtime()
+    env["LD_PRELOAD"] = emulation.sanitizer_runtime(sanitizer)
     env["ASAN_OPTIONS"] = "detect_leaks=0:abort_on_error=1"
+    env["TSAN_OPTIONS"] = "halt_on_error=1:report_signal_unsafe=0"
     for k in ["EMU_SHUFFLE", "EMU_THREADS"] + list(STRICT):
         env.pop(k, None)
     if threads is not None:
@@ -117,6 +119,14 @@ RUNS = [
     # strokes with round joins: polygons and fp32 pixels bit for bit equal
     # to the oracle's (the join's acos and tan evaluated as glibc does)
     ("round_joins", dict(names=["round_joins", "trace_stages"], threads=4)),
+    # the host runtime's threads (server workers and scheduler, batch replay
+    # threads, the caller) under ThreadSanitizer, the kernels and the emulated
+    # HIP runtime uninstrumented: a closed-loop serving burst under lowered
+    # frame bounds, a multi-canvas batch, puts in batches and server requests,
+    # and the canvas re-run path report no data race
+    ("host_races", dict(names=["serve_burst", "batch_small", "batch_and_server",
+                               "canvas_execute"], threads=4, sanitizer="thread",
+                        extra={"EMU_BATCH_DRAWS": "40", "EMU_BATCH_PIXELS": "6000"})),
     # puts recorded as packets in batch and server canvases, applied in
     # painter's order: the same pixels as the plain canvas, which applies them
     # at once
@@ -135,8 +145,15 @@ RUNS = [
 
 @pytest.fixture(scope="module")
 def runs(library):
+    libraries = {"address": library}
+    try:
+        emulation.sanitizer_runtime("thread")
+        libraries["thread"] = emulation.build(sanitizer="thread")
+    except (OSError, subprocess.CalledProcessError, RuntimeError):
+        pass  # the ThreadSanitizer run is skipped
     pool = ThreadPoolExecutor(max_workers=JOBS)
-    futures = {key: pool.submit(run, library, **kw) for key, kw in RUNS}
+    futures = {key: pool.submit(run, libraries[kw.get("sanitizer", "address")], **kw)
+               for key, kw in RUNS if kw.get("sanitizer", "address") in libraries}
     yield futures
     pool.shutdown(wait=False, cancel_futures=True)
 
@@ -190,3 +207,10 @@ def test_deferred_streams_and_a_serving_burst(runs):
 
 def test_round_joins_bit_for_bit(runs):
     runs["round_joins"].result()
+
+
+def test_host_threads_free_of_data_races(runs):
+    if "host_races" not in runs:
+        pytest.skip("ThreadSanitizer runtime not found")
+    out = runs["host_races"].result()
+    assert "ThreadSanitizer" not in out, out[-3000:]
