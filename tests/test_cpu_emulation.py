@@ -119,6 +119,9 @@ RUNS = [
     # strokes with round joins: polygons and fp32 pixels bit for bit equal
     # to the oracle's (the join's acos and tan evaluated as glibc does)
     ("round_joins", dict(names=["round_joins", "trace_stages"], threads=4)),
+    # two seeded scenes of each configuration bit for bit equal to the oracle
+    # in fp32 and sRGB8, tighter than the parity bar
+    ("bit_exact", dict(names=["bit_exact_scenes"], threads=4)),
     # the host runtime's threads (server workers and scheduler, batch replay
     # threads, the caller) under ThreadSanitizer, the kernels and the emulated
     # HIP runtime uninstrumented: a closed-loop serving burst under lowered
@@ -214,3 +217,7 @@ def test_host_threads_free_of_data_races(runs):
         pytest.skip("ThreadSanitizer runtime not found")
     out = runs["host_races"].result()
     assert "ThreadSanitizer" not in out, out[-3000:]
+
+
+def test_scenes_bit_for_bit(runs):
+    runs["bit_exact"].result()
