@@ -402,9 +402,9 @@ def test_stage_lines_and_runs(device, seed):
     stroke_lines bit for bit, and the GPU's runs, sorted and merged as the
     reference does, equal its runs.  Without the reference build, the
     oracle's stage seams stand in (tests/test_oracle.py pins them to the
-    reference's bit for bit).  Round joins are left to the end-to-end
-    tests: they go through acosf/tanf, whose last bits may differ between the
-    device and host libraries and may flip a flattening decision."""
+    reference's bit for bit).  Miter and bevel joins here; round joins
+    (acosf/tanf of the turn) are checked bit for bit in
+    test_gpu_zz_stroke_joins.py."""
     rng = np.random.default_rng(seed)
     s = Stream(160, 160)
     s.set_line_width(float(rng.uniform(1, 12))).set_line_join(seed % 2).set_line_cap(seed % 3)
