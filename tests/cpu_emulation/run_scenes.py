@@ -345,15 +345,6 @@ def round_joins(device):
         assert np.array_equal(pl.view(np.uint32), gl.view(np.uint32)), name
 
 
-# (configuration, seed, width, height, shapes): two seeded scenes per
-# configuration at small sizes
-BIT_EXACT_SCENES = [(1, 0xa00641a9f2, 124, 119, 26), (2, 0x39a72dabd6, 134, 52, 17),
-                    (3, 0xdfa132d749, 150, 48, 23), (4, 0xcc0cbdf921, 139, 57, 22),
-                    (5, 0x4d93887ad1, 86, 70, 31), (1, 0x71f08551fe, 76, 86, 24),
-                    (2, 0x8db200a195, 105, 127, 33), (3, 0x9f472352b3, 136, 75, 39),
-                    (4, 0x371e788978, 142, 60, 35), (5, 0xb3fc88f1e, 116, 83, 9)]
-
-
 def bit_exact_scenes(device):
     """Seeded scenes of every configuration render with fp32 pixels and
     sRGB8 bytes bit for bit equal to the oracle's -- tighter than the parity
@@ -361,7 +352,7 @@ def bit_exact_scenes(device):
     acosf/tanf in round joins and its read-back bytes, and sum equal-magnitude
     runs in the oracle's order (where the reference's std::sort may pick
     another, the oracle and the reference differ in the last bit)."""
-    for cfg, seed, w, h, n in BIT_EXACT_SCENES:
+    for cfg, seed, w, h, n in scenes.BIT_EXACT_SCENES:
         s = cia.scene(cfg, seed, w, h, n, 0)
         p8, pl = oracle.render(s, "port")
         g8, gl = cia.render(s, device)

@@ -898,3 +898,12 @@ def round_join_strokes():
         s.stroke()
         out["seeded_%d" % seed] = (s.tobytes(), before)
     return out
+
+
+# (configuration, seed, width, height, shapes): two seeded scenes per
+# configuration at small sizes, rendered bit for bit as the oracle renders them
+BIT_EXACT_SCENES = [(1, 0xa00641a9f2, 124, 119, 26), (2, 0x39a72dabd6, 134, 52, 17),
+                    (3, 0xdfa132d749, 150, 48, 23), (4, 0xcc0cbdf921, 139, 57, 22),
+                    (5, 0x4d93887ad1, 86, 70, 31), (1, 0x71f08551fe, 76, 86, 24),
+                    (2, 0x8db200a195, 105, 127, 33), (3, 0x9f472352b3, 136, 75, 39),
+                    (4, 0x371e788978, 142, 60, 35), (5, 0xb3fc88f1e, 116, 83, 9)]

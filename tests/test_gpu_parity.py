@@ -404,7 +404,7 @@ def test_stage_lines_and_runs(device, seed):
     oracle's stage seams stand in (tests/test_oracle.py pins them to the
     reference's bit for bit).  Miter and bevel joins here; round joins
     (acosf/tanf of the turn) are checked bit for bit in
-    test_gpu_zz_stroke_joins.py."""
+    test_gpu_zzz_stroke_joins.py."""
     rng = np.random.default_rng(seed)
     s = Stream(160, 160)
     s.set_line_width(float(rng.uniform(1, 12))).set_line_join(seed % 2).set_line_cap(seed % 3)
