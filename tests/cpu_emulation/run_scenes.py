@@ -360,6 +360,49 @@ def bit_exact_scenes(device):
         assert np.array_equal(p8, g8), (cfg, hex(seed))
 
 
+def sweep(device, n):
+    """n seeded scenes (configurations 1-5 in turn; 48-160 x 48-128 pixels,
+    8-40 shapes; the shadowed serving variant on 1 in 10 of configuration 5),
+    each with fp32 pixels and sRGB8 bytes bit for bit equal to the oracle's;
+    a line per scene also compares with the reference build where it exists."""
+    rng = np.random.default_rng(7)
+    kind = "reference" if oracle.available("reference") else None
+    for i in range(n):
+        cfg = 1 + i % 5
+        seed = int(rng.integers(1, 1 << 40))
+        w, h = int(rng.integers(48, 160)), int(rng.integers(48, 128))
+        shapes = int(rng.integers(8, 40))
+        flags = cia.SCENE_SHADOWS if (cfg == 5 and i % 10 == 4) else 0
+        s = cia.scene(cfg, seed, w, h, shapes, flags)
+        p8, pl = oracle.render(s, "port")
+        g8, gl = cia.render(s, device)
+        line = "scene %d: configuration %d seed %#x %dx%d, %d shapes" % (i, cfg, seed, w, h,
+                                                                         shapes)
+        if kind:
+            r8, rl = oracle.render(s, kind)
+            line += ", reference: %d pixels differ" % int(np.count_nonzero(
+                (gl.view(np.uint32) != rl.view(np.uint32)).any(axis=2)))
+        print(line, flush=True)
+        assert np.array_equal(pl.view(np.uint32), gl.view(np.uint32)), line
+        assert np.array_equal(p8, g8), line
+
+
+def full_config(device, cfg):
+    """A benchmark configuration at full size (seed 0xC0 + cfg): fp32 pixels
+    and sRGB8 bytes bit for bit equal to the oracle's, and the reference
+    build's where it exists (configuration 1 takes minutes, 3 hours)."""
+    s = cia.scene(cfg, 0xC0 + cfg)
+    p8, pl = oracle.render(s, "port")
+    g8, gl = cia.render(s, device)
+    differ = int(np.count_nonzero((gl.view(np.uint32) != pl.view(np.uint32)).any(axis=2)))
+    print("configuration %d: %d pixels differ from the oracle" % (cfg, differ), flush=True)
+    if oracle.available("reference"):
+        r8, rl = oracle.render(s, "reference")
+        print("configuration %d: %d pixels differ from the reference" % (cfg, int(np.count_nonzero(
+            (gl.view(np.uint32) != rl.view(np.uint32)).any(axis=2)))), flush=True)
+    assert differ == 0 and np.array_equal(p8, g8)
+
+
 def trace_stages(device):
     """cih_trace_stage reports only the stages a draw has: an undashed
     stroke's dash stage is empty, a dashed one's is not, and a rectangle
@@ -397,6 +440,12 @@ def mode(name):
     if name.startswith("batch_mix_"):
         n = int(name[len("batch_mix_"):])
         return lambda device: batch_mix(device, n)
+    if name.startswith("sweep_"):
+        n = int(name[len("sweep_"):])
+        return lambda device: sweep(device, n)
+    if name.startswith("config_"):
+        cfg = int(name[len("config_"):])
+        return lambda device: full_config(device, cfg)
     return MODES.get(name)
 
 
