@@ -357,8 +357,9 @@ def build(force=False, sanitizer="address"):
     leaves the emulated kernels and the emulated HIP runtime uninstrumented
     (their lanes switch stacks, which ThreadSanitizer does not follow): it
     checks the host code's threads (server workers and scheduler, batch
-    replay threads, callers) for data races, not the kernels."""
-    out = OUT if sanitizer == "address" else OUT + "_" + sanitizer
+    replay threads, callers) for data races, not the kernels.  sanitizer=None
+    (OUT_plain/...) instruments nothing, for timing host code."""
+    out = OUT if sanitizer == "address" else OUT + "_" + (sanitizer or "plain")
     library = os.path.join(out, "libcanvas_ity_hip.so")
     os.makedirs(out, exist_ok=True)
     stamp = os.path.join(out, "sources.sha256")
@@ -379,8 +380,8 @@ def build(force=False, sanitizer="address"):
         f.write(runtime)
     flags = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-fno-omit-frame-pointer",
              "-ffp-contract=off", "-I" + HERE, "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
-    host = ["-fsanitize=" + sanitizer]
-    emulated = host if sanitizer == "address" else []
+    host = ["-fsanitize=" + sanitizer] if sanitizer else ["-O2"]
+    emulated = host if sanitizer != "thread" else []
     jobs = [
         (os.path.join(out, "emu_kernels.cpp"), emulated),
         (os.path.join(out, "emu_runtime.cpp"), host + ["-fpermissive", "-fvisibility=hidden",
@@ -403,8 +404,9 @@ def build(force=False, sanitizer="address"):
             raise RuntimeError("emulation build failed:\n" + text.decode()[-4000:])
     # linked beside and renamed into place: a process that has the old
     # library loaded keeps it
-    subprocess.run(["g++", "-shared", "-fsanitize=" + sanitizer, "-o", library + ".tmp"] +
-                   objects + ["-lpthread"], check=True)
+    link = ["-fsanitize=" + sanitizer] if sanitizer else []
+    subprocess.run(["g++", "-shared"] + link + ["-o", library + ".tmp"] + objects +
+                   ["-lpthread"], check=True)
     os.replace(library + ".tmp", library)
     with open(stamp, "w") as f:
         f.write(digest)
@@ -413,4 +415,5 @@ def build(force=False, sanitizer="address"):
 
 if __name__ == "__main__":
     print(build(force="--force" in sys.argv,
-                sanitizer="thread" if "--thread" in sys.argv else "address"))
+                sanitizer="thread" if "--thread" in sys.argv else
+                None if "--plain" in sys.argv else "address"))

@@ -352,8 +352,8 @@ def bit_exact_scenes(device):
     acosf/tanf in round joins and its read-back bytes, and sum equal-magnitude
     runs in the oracle's order (where the reference's std::sort may pick
     another, the oracle and the reference differ in the last bit)."""
-    for cfg, seed, w, h, n in scenes.BIT_EXACT_SCENES:
-        s = cia.scene(cfg, seed, w, h, n, 0)
+    for cfg, seed, w, h, n, flags in scenes.BIT_EXACT_SCENES:
+        s = cia.scene(cfg, seed, w, h, n, flags)
         p8, pl = oracle.render(s, "port")
         g8, gl = cia.render(s, device)
         assert np.array_equal(pl.view(np.uint32), gl.view(np.uint32)), (cfg, hex(seed))

@@ -900,10 +900,12 @@ def round_join_strokes():
     return out
 
 
-# (configuration, seed, width, height, shapes): two seeded scenes per
-# configuration at small sizes, rendered bit for bit as the oracle renders them
-BIT_EXACT_SCENES = [(1, 0xa00641a9f2, 124, 119, 26), (2, 0x39a72dabd6, 134, 52, 17),
-                    (3, 0xdfa132d749, 150, 48, 23), (4, 0xcc0cbdf921, 139, 57, 22),
-                    (5, 0x4d93887ad1, 86, 70, 31), (1, 0x71f08551fe, 76, 86, 24),
-                    (2, 0x8db200a195, 105, 127, 33), (3, 0x9f472352b3, 136, 75, 39),
-                    (4, 0x371e788978, 142, 60, 35), (5, 0xb3fc88f1e, 116, 83, 9)]
+# (configuration, seed, width, height, shapes, flags): two seeded scenes per
+# configuration at small sizes and two of configuration 2 with even-odd fills
+# (flags 1), rendered bit for bit as the oracle renders them
+BIT_EXACT_SCENES = [(1, 0xa00641a9f2, 124, 119, 26, 0), (2, 0x39a72dabd6, 134, 52, 17, 0),
+                    (3, 0xdfa132d749, 150, 48, 23, 0), (4, 0xcc0cbdf921, 139, 57, 22, 0),
+                    (5, 0x4d93887ad1, 86, 70, 31, 0), (1, 0x71f08551fe, 76, 86, 24, 0),
+                    (2, 0x8db200a195, 105, 127, 33, 0), (3, 0x9f472352b3, 136, 75, 39, 0),
+                    (4, 0x371e788978, 142, 60, 35, 0), (5, 0xb3fc88f1e, 116, 83, 9, 0),
+                    (2, 0x20e9fa1023, 140, 95, 27, 1), (2, 0xed9f3cd366, 116, 68, 26, 1)]
