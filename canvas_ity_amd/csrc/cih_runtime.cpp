@@ -1624,14 +1624,12 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
                 const draw_desc &dd = b->descs[last];
                 if (dd.canvas != first_canvas && last - draw >= kBatchDrawLimit)
                     break;
-                if (dd.canvas != last_canvas) {
-                    uint64_t px = b->pixel_prefix[(size_t)dd.canvas + 1] -
-                                  b->pixel_prefix[(size_t)dd.canvas];
-                    if (last > draw && pixels + px > kBatchPixelLimit)
-                        break;
-                    pixels += px;
-                    last_canvas = dd.canvas;
-                }
+                bool new_canvas = dd.canvas != last_canvas;
+                uint64_t px = new_canvas ? b->pixel_prefix[(size_t)dd.canvas + 1] -
+                                               b->pixel_prefix[(size_t)dd.canvas]
+                                         : 0;
+                if (last > draw && pixels + px > kBatchPixelLimit)
+                    break;
                 if (frame_cut(dd, dd.canvas, used, current) && last > draw)
                     break;
                 uint64_t need = 0;
@@ -1643,6 +1641,9 @@ int cih_batch_create(cih_device *d, int count, const void *const *streams, const
                 if (last > draw && shadow + need > shadow_budget)
                     break;
                 shadow += need;
+                pixels += px;
+                if (new_canvas)
+                    last_canvas = dd.canvas;
                 ++last;
             }
             cih_batch::frame_rec f;
