@@ -185,7 +185,8 @@ CIH_API int cih_device_profile_read(cih_device *device, char *json, size_t capac
 
 /* Test seams for stage-level differential tests.  They read the last frame
  * run on the canvas' device, so call them right after a flush whose draws fit
- * one frame and before any other canvas on the device is flushed.
+ * one frame, before any other canvas on the device is flushed and while the
+ * canvas that ran it is alive (its draw records are read).
  * cih_trace_polygons: the closed polygons draw `draw` produced (flattening,
  * dashing and stroking output, device space) -- the reference's `lines`
  * after path_to_lines/stroke_lines (canvas_ity.hpp:1495, 2070).
