@@ -441,6 +441,7 @@ struct emu_event {
     emu_stream *stream = nullptr;
     uint64_t at = 0; // the stream's queued count when recorded
     bool queried = false;
+    double seconds = 0; // wall clock when the stream reached the event
 };
 static int g_err = hipSuccess;
 static std::mutex streams_mutex;
@@ -623,6 +624,11 @@ hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t st) {
     e->stream = st;
     e->queried = false;
+    // the time is taken in stream order (at once without EMU_ASYNC_STREAMS)
+    emu::launch(st, [e]() {
+        e->seconds = std::chrono::duration<double>(
+                         std::chrono::steady_clock::now().time_since_epoch()).count();
+    });
     if (st) {
         std::lock_guard<std::mutex> lock(st->m);
         e->at = st->queued;
@@ -644,7 +650,10 @@ hipError_t hipEventSynchronize(hipEvent_t e) {
         drain_stream(e->stream, e->at);
     return hipSuccess;
 }
-hipError_t hipEventElapsedTime(float *ms, hipEvent_t, hipEvent_t) { *ms = 0.0f; return hipSuccess; }
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t start, hipEvent_t stop) {
+    *ms = (float)((stop->seconds - start->seconds) * 1000.0);
+    return hipSuccess;
+}
 hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorNotSupported; }
 hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *) { return hipErrorNotSupported; }
 hipError_t hipGraphInstantiate(hipGraphExec_t *, hipGraph_t, void *, char *, size_t) { return hipErrorNotSupported; }
