@@ -98,8 +98,9 @@ def cpu_baseline(cia, stream, width, height, sample_draws, threads, runs=5):
         "value": round(width * height / 1e6 / single, 4), "unit": "Mpix/s", "cores": 1,
         "kind": kind, "cpu": model,
         "sample": "large-2048 stream %#x: first %d of %d draws, median of %d runs after a "
-                  "warm-up, one thread, scaled x%.3g to the whole canvas (%.2f s per canvas)"
-                  % (SEED, sample_draws, total, runs, scale, single),
+                  "warm-up, one thread, scaled x%.3g to the whole canvas (%.2f s per canvas); "
+                  "sampled because five whole canvases and the per-core run would take "
+                  "several minutes of the bench" % (SEED, sample_draws, total, runs, scale, single),
         "per_core": {"value": round(threads * width * height / 1e6 / wall, 4),
                      "unit": "Mpix/s", "cores": threads,
                      "sample": "%d concurrent instances of the same sample, one per thread, "
